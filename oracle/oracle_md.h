@@ -1,0 +1,111 @@
+/*
+ * CPU oracle for the MD-Bench short-range force step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A strict-IEEE (no FMA contraction, no fast-math) C restatement of the reference algorithm,
+ * used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  The
+ * product (md-bench_amd/) never links, loads or calls anything here.
+ *
+ * Every function cites the reference file:line it follows.  Pinning (what checks each function
+ * against something independent of this file):
+ *   orc_vl_setup_grid / orc_vl_coord2bin / orc_vl_binatoms   golden bins (tests/golden/vl_*.npz)
+ *   orc_vl_setup_pbc / orc_vl_update_pbc                     golden ghost map and ghost x
+ *   orc_vl_build_neighbor                                    golden lists (full and half)
+ *   orc_vl_force_lj                                          golden forces; closed forms; brute force
+ *   orc_vl_force_lj energy / virial                          finite differences (-dE/dx = f), virial
+ *                                                            theorem on pair sums (no reference term)
+ *   orc_eam_*                                                golden spline tables and EAM forces;
+ *                                                            finite-difference -dE/dx
+ *   orc_integrate_*                                          closed forms of tests/test_integrate.c
+ *   orc_vl_run                                               golden trajectories; T/P at step 200
+ *   orc_cp_*                                                 see oracle_cp.c header
+ */
+#ifndef ORACLE_MD_H
+#define ORACLE_MD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bin grid of src/verletlist/neighbor.c:60-258 (file-scope state there). */
+typedef struct {
+    double xprd, yprd, zprd;
+    double cutneigh, cutneighsq;
+    int nbinx, nbiny, nbinz;
+    double binsizex, binsizey, binsizez;
+    double bininvx, bininvy, bininvz;
+    int pad_x, pad_y, pad_z;
+    int mbinx, mbiny, mbinz, mbins;
+    int nstencil;
+    int stencil[1024];
+} orc_vl_grid;
+
+int orc_vl_setup_grid(orc_vl_grid* g, int nx, int ny, int nz, double lattice, double cutneigh);
+int orc_vl_coord2bin(const orc_vl_grid* g, double x, double y, double z);
+/* Bins all nall atoms; bins is CSR (binstart[mbins+1], binatoms[nall]) in atom-index order. */
+void orc_vl_binatoms(const orc_vl_grid* g, int nall, const double* x, int* binstart, int* binatoms);
+/* Neighbour list as CSR: numneigh[nlocal], neighbors[cap]. Returns total count or -1 if cap
+   exceeded. */
+long long orc_vl_build_neighbor(const orc_vl_grid* g, int nlocal, int nall, const double* x,
+    const int* type, int ntypes, const double* cutneighsq, int half, int* numneigh,
+    int* neighbors, long long cap);
+
+/* Ghost map of src/verletlist/pbc.c:103-232. Returns Nghost or -1 if cap exceeded. */
+int orc_vl_setup_pbc(int nlocal, const double* x, double xprd, double yprd, double zprd,
+    double cutneigh, int* border_map, int* pbc, int cap);
+void orc_vl_update_pbc(int nlocal, int nghost, double* x, const int* border_map, const int* pbc,
+    double xprd, double yprd, double zprd);
+void orc_update_atoms_pbc(int nlocal, double* x, double xprd, double yprd, double zprd);
+
+/* LJ 12-6 on a CSR list (src/verletlist/force_lj.c:17-206). f has nall*3 entries (half list
+   writes local j only, like the reference). energy/virial (may be NULL): sum over evaluated
+   pairs of 4 eps (sr12 - sr6) and r.F, halved for full lists so both modes report the
+   per-system totals. */
+void orc_vl_force_lj(int nlocal, int nall, const double* x, const int* type, int ntypes,
+    const double* epsilon, const double* sigma6, const double* cutforcesq, const int* numneigh,
+    const int* neighbors, int half, double* f, double* energy, double* virial);
+
+/* EAM tables (src/common/eam_utils.c). */
+typedef struct {
+    int nrho, nr;
+    double drho, dr, cut, mass;
+    double *frho, *rhor, *zr; /* 1-based as readEamFile stores them */
+} orc_funcfl;
+typedef struct {
+    int nrho, nr, nrho_tot, nr_tot;
+    double dr, rdr, drho, rdrho;
+    double *frho_spline, *rhor_spline, *z2r_spline; /* one type pair */
+} orc_eam;
+
+int orc_eam_read_funcfl(const char* path, orc_funcfl* file);
+int orc_eam_setup(const orc_funcfl* file, orc_eam* eam);
+void orc_eam_free(orc_funcfl* file, orc_eam* eam);
+/* EAM force (src/verletlist/force_eam.c:19-228), single pair table. fp[nall] is filled for
+   locals and copied to ghosts through border_map.  energy (may be NULL) = sum F(rho_i) + half
+   pair sums. */
+void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* x,
+    const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
+    double* fp, double* f, double* energy);
+
+/* Velocity Verlet (src/verletlist/integrate.c:21-42). */
+void orc_initial_integrate(int n, double* x, double* v, const double* f, double dt,
+    double dtforce);
+void orc_final_integrate(int n, double* v, const double* f, double dtforce);
+
+/* Sum of m v^2 over n atoms, serial order (src/common/thermo.c:76-81). */
+double orc_kinetic_sum(int n, const double* v, double mass);
+
+/* Whole Verlet-list trajectory: setup from given x,v,type (already adjusted), then the loop of
+   src/verletlist/main.c:371-419.  ksum[ntimes+1] receives sum m v^2 after every step.
+   force_field 0 = LJ, 1 = EAM (eam must be set).  Returns 0 on success. */
+int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
+    const int* type, int ntypes, const double* epsilon, const double* sigma6,
+    const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
+    int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
+    int reneigh_every, double mass, double* f, double* ksum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

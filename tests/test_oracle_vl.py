@@ -1,0 +1,118 @@
+"""The CPU oracle (Verlet-list path) against the compiled reference's golden vectors.
+
+Golden vectors come from MD-Bench's own Verlet-list sources built with strict IEEE flags
+(tests/golden/make_golden.py).  Bins, ghost maps and neighbour lists are integer work and must be
+identical; forces and trajectories of the strict restatement must be bit-identical too.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import load_golden, meta
+
+LJ_CASES = ["vl_lj_4", "vl_lj_4_half", "vl_lj_5x6x7", "vl_lj_types"]
+EAM_CASES = ["vl_eam_synth_4", "vl_eam_cu_u3_4"]
+
+
+def _dims(d):
+    args = [str(a) for a in d["args"]]
+    return int(args[0]), int(args[1]), int(args[2]), int(args[3])
+
+
+def _grid(d):
+    nx, ny, nz, _ = _dims(d)
+    return ob.vl_grid(nx, ny, nz, meta(d, "lattice"), meta(d, "cutneigh"))
+
+
+@pytest.mark.parametrize("case", LJ_CASES + EAM_CASES)
+def test_grid_matches_reference(case):
+    d = load_golden(case)
+    g = _grid(d)
+    for k in ["mbinx", "mbiny", "mbinz", "pad_x", "pad_y", "pad_z", "nstencil"]:
+        assert getattr(g, k) == meta(d, k), k
+    for k in ["binsizex", "binsizey", "binsizez", "bininvx", "bininvy", "bininvz"]:
+        assert getattr(g, k) == meta(d, k), k
+    assert list(g.stencil[:g.nstencil]) == list(d["stencil"])
+
+
+@pytest.mark.parametrize("case", LJ_CASES + EAM_CASES)
+def test_ghost_map_bins_and_lists(case):
+    d = load_golden(case)
+    g = _grid(d)
+    nlocal = meta(d, "s0_nlocal")
+    x0 = d["s0_x"][:3 * nlocal]
+    bmap, pbc = ob.setup_pbc(nlocal, x0, (g.xprd, g.yprd, g.zprd), g.cutneigh)
+    assert bmap.size == meta(d, "s0_nghost")
+    np.testing.assert_array_equal(bmap, d["s0_border_map"])
+    np.testing.assert_array_equal(pbc, d["s0_pbc"])
+    x = np.zeros(3 * (nlocal + bmap.size))
+    x[:3 * nlocal] = x0
+    x = ob.update_pbc(nlocal, x, bmap, pbc, (g.xprd, g.yprd, g.zprd))
+    np.testing.assert_array_equal(x, d["s0_x"])  # ghost positions bit-identical
+    start, atoms = ob.binatoms(g, x)
+    np.testing.assert_array_equal(np.diff(start), d["s0_bincount"])
+    np.testing.assert_array_equal(atoms, d["s0_bins"])
+    half = _dims(d)[3]
+    numneigh, nb = ob.build_neighbor(g, nlocal, x, d["s0_type"], meta(d, "ntypes"),
+                                     d["cutneighsq"], half)
+    np.testing.assert_array_equal(numneigh, d["s0_numneigh"])
+    np.testing.assert_array_equal(nb, d["s0_neighbors"])  # same order, not only same sets
+
+
+@pytest.mark.parametrize("case", LJ_CASES)
+def test_lj_force_bit_identical(case):
+    d = load_golden(case)
+    nlocal = meta(d, "s0_nlocal")
+    half = _dims(d)[3]
+    f, e, w = ob.force_lj(nlocal, d["s0_x"], d["s0_type"], meta(d, "ntypes"), d["epsilon"],
+                          d["sigma6"], d["cutforcesq"], d["s0_numneigh"], d["s0_neighbors"], half)
+    np.testing.assert_array_equal(f[:3 * nlocal], d["s0_f"])
+
+
+@pytest.mark.parametrize("case", LJ_CASES + EAM_CASES)
+def test_trajectory_matches_reference(case):
+    d = load_golden(case)
+    nx, ny, nz, half = _dims(d)
+    args = [str(a) for a in d["args"]]
+    ntimes, reneigh = int(args[4]), int(args[5])
+    tab = None
+    if args[6] == "eam":
+        tab = ob.EamTables(_eam_path(args[7]))
+    x, v, f, ks = ob.vl_run(nx, ny, nz, meta(d, "lattice"), d["create_x"], d["adjust_v"],
+                            d["create_type"], meta(d, "ntypes"), d["epsilon"], d["sigma6"],
+                            d["cutforcesq"], d["cutneighsq"], meta(d, "cutneigh"), half, ntimes,
+                            reneigh, meta(d, "dt"), meta(d, "dtforce"), meta(d, "mass"), eam=tab)
+    nlocal = meta(d, "end_nlocal")
+    np.testing.assert_array_equal(ks, d["traj_ksum"])
+    np.testing.assert_array_equal(x, d["end_x"][:3 * nlocal])
+    np.testing.assert_array_equal(v, d["end_v"])
+    np.testing.assert_array_equal(f, d["end_f"])
+
+
+def _eam_path(arg):
+    """EAM tables: the synthetic table is regenerated here; Cu_u3 comes from its fixture copy."""
+    import tempfile
+    from golden.eam_tables import write_funcfl_synthetic
+    if "Cu_u3" in arg:
+        return os.path.join(os.path.dirname(__file__), "golden", "Cu_u3.eam")
+    path = os.path.join(tempfile.gettempdir(), "mdb_cu_synth_%d.eam" % os.getpid())
+    write_funcfl_synthetic(path)
+    return path
+
+
+@pytest.mark.parametrize("case", EAM_CASES)
+def test_eam_tables_and_force(case):
+    d = load_golden(case)
+    args = [str(a) for a in d["args"]]
+    tab = ob.EamTables(_eam_path(args[7]))
+    assert tab.eam.nr == meta(d, "eam_nr") and tab.eam.nrho == meta(d, "eam_nrho")
+    assert tab.eam.nr_tot == meta(d, "eam_nr_tot")
+    for w in ["frho", "rhor", "z2r"]:
+        np.testing.assert_array_equal(tab.spline(w), d["eam_%s_spline" % w])
+    nlocal = meta(d, "s0_nlocal")
+    cutf = d["cutforcesq"][0]
+    f, fp, e = ob.force_eam(tab, nlocal, d["s0_x"], d["s0_border_map"], cutf, d["s0_numneigh"],
+                            d["s0_neighbors"])
+    np.testing.assert_array_equal(f, d["s0_f"])
