@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the MD-Bench short-range force step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line.  A step is one velocity-Verlet MD timestep of the whole system (integrate,
+ghost update, force, integrate; a neighbour rebuild every reneigh_every steps), with every array
+already resident in HBM.  `value` = atoms x steps / wall time of the K timed steps.
+
+Workloads (seeded fcc lattices, synthetic velocities):
+  vl-lj-64   Verlet-list LJ, FP64, full list, 64^3 cells = 1,048,576 atoms, rho* 0.8442,
+             r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
+
+`roofline` covers the dominant kernel (the force kernel): algorithmic bytes per launch over its
+average launch time, both measured inside the timed region with HIP events on the library stream.
+`cpu_baseline` times MD-Bench itself (built from its sources into oracle/_ref) on the host cores
+on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+METRIC = "atom-steps/s (LJ, EAM) + surrogate/plain EAM speedup at fixed force RMSE"
+HBM_PEAK_GBS = 8000.0
+
+
+def binding():
+    import importlib.util
+    path = os.path.join(REPO, "md-bench_amd", "mdbench_gfx950.py")
+    spec = importlib.util.spec_from_file_location("mdbench_gfx950", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mdbench_gfx950"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except Exception:
+        pass
+    return set()
+
+
+def run_reference(binary, args, cores):
+    """Time MD-Bench's own binary; returns atom-steps/s from its 'atom updates/us' line."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_SCHEDULE="static",
+               OMP_PROC_BIND="close")
+    out = subprocess.run([binary] + args, env=env, capture_output=True, text=True, timeout=300,
+                         cwd="/tmp")
+    m = re.search(r"([0-9.]+) atom updates/us", out.stdout)
+    if out.returncode != 0 or not m:
+        raise RuntimeError("reference run failed: %s" % out.stdout[-500:])
+    return float(m.group(1)) * 1e6
+
+
+def cpu_baseline_vl(nx, steps):
+    cores = host_cores()
+    ref = os.path.join(REPO, "oracle", "_ref", "mdbench-vl-fast")
+    sample = "%d^3 cells (%d atoms), %d timesteps, full list, MD-Bench -Ofast scalar FP64" % (
+        nx, 4 * nx ** 3, steps)
+    if os.path.exists(ref):
+        v = run_reference(ref, ["-nx", str(nx), "-ny", str(nx), "-nz", str(nx), "-n",
+                                str(steps)], cores)
+        return {"value": v, "unit": "atom-steps/s", "cores": cores, "kind": "reference",
+                "sample": sample}
+    return None
+
+
+def bench_vl(args, mod):
+    nx = 64
+    sim = mod.VLSim(nx=nx, ny=nx, nz=nx, ntimes=args.steps + args.warmup)
+    try:
+        natoms = sim.atom.Natoms
+        sim.run(args.warmup)
+        L = mod.lib()
+        L.mdgfx950Synchronize()
+        L.mdgfx950EnableTiming(1)
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        L.mdgfx950Synchronize()
+        t1 = time.perf_counter()
+        import ctypes as C
+        ms = C.c_double()
+        nl = C.c_longlong()
+        L.mdgfx950KernelTime(b"force_lj", C.byref(ms), C.byref(nl))
+        L.mdgfx950EnableTiming(0)
+        st = sim.state()
+        sum_nn = int(st["numneigh"].sum())
+        nlocal = st["nlocal"]
+        elapsed = t1 - t0
+        # Algorithmic bytes of one full-list FP64 force launch (MD-Bench's data-volume model,
+        # src/verletlist/stats.c:56-59): per atom x_i (24 B) + f_i (24 B) + numneigh (4 B),
+        # per neighbour index (4 B) + x_j (24 B).
+        bytes_launch = nlocal * 52.0 + sum_nn * 28.0
+        avg_ms = ms.value / max(nl.value, 1)
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        res = {
+            "metric": METRIC, "value": natoms * args.steps / elapsed, "unit": "atom-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "vl-lj-64: Verlet-list LJ FP64 full list, 64^3 fcc cells "
+                                   "(%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20" % natoms,
+                       "atoms": natoms, "avg_neighbors": sum_nn / nlocal},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_force_lj (fast, 4 lanes/atom)",
+                         "avg_launch_ms": avg_ms, "launches": nl.value,
+                         "bytes_per_launch": bytes_launch},
+        }
+        return res, nx
+    finally:
+        sim.close()
+
+
+WORKLOADS = {"vl-lj-64": bench_vl}
+DEFAULT = "vl-lj-64"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; a multi-GPU (spatial domain "
+              "decomposition) run is not part of it. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    mod = binding()
+    res, nx = WORKLOADS[args.workload](args, mod)
+    cb = None
+    if not args.no_cpu_baseline:
+        try:
+            cb = cpu_baseline_vl(nx, 20)
+        except Exception as exc:  # the GPU figure stands on its own
+            print("cpu baseline failed: %s" % exc, file=sys.stderr)
+    res["cpu_baseline"] = cb
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+/*
+ * libmdgfx950 -- MI355X (gfx950) short-range force step for MD-Bench.  C ABI.
+ *
+ * The Verlet-list entry points take MD-Bench's own structs (mdbench_vl.h) and keep its
+ * signatures, so MD-Bench's host code binds them exactly where it binds its CUDA variants:
+ * the function-pointer globals computeForce (src/verletlist/force.c:13-42), buildNeighbor
+ * (src/verletlist/neighbor.c:23-27), updatePbc/updateAtomsPbc (src/verletlist/pbc.c:20-26) and
+ * initialIntegrate/finalIntegrate (src/verletlist/integrate.c:13-19).  All state stays on the
+ * GPU; the host copies are touched only by copyDataFromGFX950Device.
+ *
+ * Differences from the reference's conventions, by design:
+ *   - no exit() inside the library: failures set a sticky status (mdgfx950Status) and print one
+ *     line to stderr; entry points become no-ops until mdgfx950ClearStatus;
+ *   - calls are asynchronous on the library's stream; the double returned by computeForce* is
+ *     host seconds spent in the call (enqueue time).  Kernel time comes from
+ *     mdgfx950KernelTime when timing is enabled;
+ *   - like the reference's file-static device state, one simulation per process at a time.
+ */
+#ifndef MDGFX950_H
+#define MDGFX950_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#include "mdbench_vl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status, mode, timing (new) */
+enum {
+    MDG_OK            = 0,
+    MDG_ERR_HIP       = 1, /* a HIP runtime call failed */
+    MDG_ERR_BIN       = 2, /* an atom fell outside the bin grid */
+    MDG_ERR_CAPACITY  = 3, /* a device buffer could not be grown */
+    MDG_ERR_ARG       = 4, /* invalid argument or call order */
+    MDG_ERR_NODEVICE  = 5
+};
+int mdgfx950Status(void);
+const char* mdgfx950StatusMessage(void);
+void mdgfx950ClearStatus(void);
+/* 1 = strict kernels (one lane per atom, reference summation order, no FMA): FP64 results
+   bit-identical to the reference; 0 = fast kernels (default). */
+void mdgfx950SetStrict(int strict);
+void mdgfx950EnableTiming(int on);
+/* Sum of kernel time (ms) and launch count of one kernel family since timing was enabled.
+   Families: "force_lj", "force_eam", "neighbor", "binning", "pbc", "integrate", "thermo". */
+int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches);
+/* Wait for all queued work. */
+void mdgfx950Synchronize(void);
+
+/* ------------------------------------------------------------ Verlet-list scheme (drop-in) */
+/* replaces initDevice (src/verletlist/device_spec.c:12-52) */
+void initDeviceGFX950(Parameter* param, Atom* atom, Neighbor* neighbor);
+/* releases everything initDeviceGFX950 and later calls allocated (no reference counterpart) */
+void freeDeviceGFX950(Atom* atom, Neighbor* neighbor);
+/* replace copyDataToCUDADevice / copyDataFromCUDADevice (src/verletlist/forceCuda.cu:327-345):
+   x, v, type of Nlocal atoms; From also copies f */
+void copyDataToGFX950Device(Parameter* param, Atom* atom);
+void copyDataFromGFX950Device(Parameter* param, Atom* atom);
+/* replaces setupNeighbor (src/verletlist/neighbor.c:101-258): bin grid and stencil */
+void setupNeighborGFX950(Parameter* param);
+/* replaces binatoms (src/verletlist/neighbor.c:431-464): counting sort, atoms in index order
+   inside each bin */
+void binatomsGFX950(Atom* atom);
+/* replaces buildNeighborCUDA (src/verletlist/neighborCuda.cu:259-366) / buildNeighborCPU
+   (src/verletlist/neighbor.c:260-352); grows neighbor->maxneighs on overflow */
+void buildNeighborGFX950(Atom* atom, Neighbor* neighbor);
+/* replaces setupPbc (src/verletlist/pbc.c:103-232), on the device: same ghost order */
+void setupPbcGFX950(Atom* atom, Parameter* param);
+/* replace updatePbcCUDA / updateAtomsPbcCUDA (src/verletlist/pbcCuda.cu:78-142) */
+void updatePbcGFX950(Atom* atom, Parameter* param, bool doReneighbor);
+void updateAtomsPbcGFX950(Atom* atom, Parameter* param, bool doReneighbor);
+/* replaces computeForceLJCUDA (src/verletlist/forceCuda.cu:227-303): full or half list by
+   neighbor->half_neigh */
+double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
+/* replaces computeForceEam (src/verletlist/force_eam.c:19-228); tables from setEamTablesGFX950 */
+double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
+/* uploads one pair table in the reference's 7-coefficient layout (src/common/eam.h:20-29,
+   eam_utils.c:222-284); replaces the global `Eam eam` the reference kernel reads */
+void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double* rhor_spline,
+    const double* frho_spline, const double* z2r_spline, int nr_tot, int nrho_tot);
+/* replace initialIntegrateCUDA / finalIntegrateCUDA (src/verletlist/forceCuda.cu:192-225) */
+void initialIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);
+void finalIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);
+
+/* ------------------------------------------------------------ additions (no counterpart) */
+/* sum over local atoms of mass * |v|^2 (the reduction inside computeThermo,
+   src/common/thermo.c:76-81), reduced on the device in FP64 */
+double kineticSumGFX950(Parameter* param, Atom* atom);
+/* potential energy and virial sum (sum r.F) of the last computeForce*GFX950 call, FP64 */
+void energyVirialGFX950(double* epot, double* virial);
+/* ask the next force calls to accumulate energy and virial (off by default) */
+void requestEnergyVirialGFX950(int on);
+/* device -> host copies for tests: positions of all Nlocal+Nghost atoms, ghost map, bins,
+   neighbour list (as CSR), EAM fp */
+int downloadVLStateGFX950(Atom* atom, Neighbor* neighbor, double* x, double* f, double* v,
+    int* border_map, int* pbc, int* numneigh, int* neighbors_csr, long long csr_cap);
+int downloadBinsGFX950(int* binstart, int mbins_cap, int* binatoms, int nall_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

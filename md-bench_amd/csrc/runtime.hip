@@ -1,0 +1,316 @@
+// Runtime pieces shared by every gfx950 entry point.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <mutex>
+#include <vector>
+
+#include "mdgfx950.h"
+#include "runtime.h"
+
+namespace mdg {
+
+// ---------------------------------------------------------------- status
+static int g_status = 0;
+static char g_msg[512];
+
+void set_error(int code, const char* fmt, ...)
+{
+    if (g_status) return;  // keep the first failure
+    g_status = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "libmdgfx950: %s\n", g_msg);
+}
+
+bool failed() { return g_status != 0; }
+
+// ---------------------------------------------------------------- stream and mode
+static hipStream_t g_stream = nullptr;
+static bool g_strict        = false;
+
+hipStream_t stream()
+{
+    if (!g_stream) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            set_error(MDG_ERR_NODEVICE, "no HIP device visible");
+            return nullptr;
+        }
+        // Default priority on purpose: streams of mixed priorities are avoided.
+        MDG_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    }
+    return g_stream;
+}
+
+bool strict_mode() { return g_strict; }
+
+double wtime()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+// ---------------------------------------------------------------- timing
+struct Pending {
+    hipEvent_t a, b;
+    int fam;
+};
+static bool g_timing = false;
+static double g_ms[F_NFAMILY];
+static long long g_launch[F_NFAMILY];
+static std::vector<Pending> g_pending;
+static std::vector<hipEvent_t> g_pool;
+static const char* g_famname[F_NFAMILY] = { "force_lj", "force_eam", "neighbor", "binning", "pbc",
+    "integrate", "thermo", "force_cp", "cluster", "surrogate" };
+
+static hipEvent_t take_event()
+{
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    MDG_HIP(hipEventCreate(&e));
+    return e;
+}
+
+static void harvest()
+{
+    for (auto& p : g_pending) {
+        float ms = 0.f;
+        MDG_HIP(hipEventSynchronize(p.b));
+        MDG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        g_ms[p.fam] += ms;
+        g_launch[p.fam] += 1;
+        g_pool.push_back(p.a);
+        g_pool.push_back(p.b);
+    }
+    g_pending.clear();
+}
+
+TimedRegion::TimedRegion(Family f) : fam(f), slot(-1)
+{
+    if (!g_timing || failed()) return;
+    Pending p;
+    p.a   = take_event();
+    p.b   = take_event();
+    p.fam = f;
+    MDG_HIP(hipEventRecord(p.a, stream()));
+    g_pending.push_back(p);
+    slot = (int)g_pending.size() - 1;
+}
+
+TimedRegion::~TimedRegion()
+{
+    if (slot < 0) return;
+    MDG_HIP(hipEventRecord(g_pending[slot].b, stream()));
+    if (g_pending.size() >= 1024) harvest();
+}
+
+// ---------------------------------------------------------------- buffers
+bool grow(void** ptr, size_t* cap, size_t bytes, size_t keep)
+{
+    if (failed()) return false;
+    if (bytes <= *cap && *ptr) return true;
+    size_t nb = bytes + bytes / 4 + 256;
+    void* np  = nullptr;
+    if (hipMalloc(&np, nb) != hipSuccess) {
+        set_error(MDG_ERR_CAPACITY, "hipMalloc of %zu bytes failed", nb);
+        return false;
+    }
+    if (*ptr) {
+        if (keep) MDG_HIP(hipMemcpyAsync(np, *ptr, keep < *cap ? keep : *cap,
+                      hipMemcpyDeviceToDevice, stream()));
+        MDG_HIP(hipStreamSynchronize(stream()));
+        MDG_HIP(hipFree(*ptr));
+    }
+    *ptr = np;
+    *cap = nb;
+    return true;
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------- fills
+template <class T>
+__global__ void k_fill(T* __restrict__ p, T v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <class T>
+static void fill_t(T* p, T v, size_t n)
+{
+    if (failed() || n == 0) return;
+    unsigned g = blocks_for((long long)n, 256);
+    if (g > 8192) g = 8192;
+    k_fill<T><<<g, 256, 0, stream()>>>(p, v, n);
+    MDG_LAUNCH_CHECK("fill");
+}
+void fill_i32(int* p, int v, size_t n) { fill_t<int>(p, v, n); }
+void fill_f64(double* p, double v, size_t n) { fill_t<double>(p, v, n); }
+void fill_f32(float* p, float v, size_t n) { fill_t<float>(p, v, n); }
+
+// ---------------------------------------------------------------- exclusive scan
+// Tiles of 256 threads x 8 items.  Pass 1 sums tiles, pass 2 scans tile sums in one block,
+// pass 3 rescans each tile with its offset.
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__device__ inline int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across the block; returns the block total in *tot.
+__device__ inline int block_excl_scan(int v, int* sh, int* tot)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        int s  = lane < nw ? sh[lane] : 0;
+        int si = wave_incl_scan(s);
+        if (lane < nw) sh[lane] = si - s;
+        if (lane == nw - 1) sh[32] = si;
+    }
+    __syncthreads();
+    int r = inc - v + sh[w];
+    *tot  = sh[32];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ tile_sums)
+{
+    __shared__ int sh[33];
+    long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
+    int s          = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_I; k++)
+        if (base + k < n) s += in[base + k];
+    int tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __restrict__ d_total)
+{
+    __shared__ int sh[33];
+    // each thread owns a contiguous chunk of tile sums
+    int per   = (ntiles + blockDim.x - 1) / blockDim.x;
+    int begin = threadIdx.x * per;
+    int s     = 0;
+    for (int k = 0; k < per; k++)
+        if (begin + k < ntiles) s += tile_sums[begin + k];
+    int tot;
+    int off = block_excl_scan(s, sh, &tot);
+    for (int k = 0; k < per; k++)
+        if (begin + k < ntiles) {
+            int t                  = tile_sums[begin + k];
+            tile_sums[begin + k]   = off;
+            off += t;
+        }
+    if (threadIdx.x == 0 && d_total) *d_total = tot;
+}
+
+__global__ void k_scan_apply(const int* __restrict__ in, int* __restrict__ out, int n,
+    const int* __restrict__ tile_sums)
+{
+    __shared__ int sh[33];
+    long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
+    int v[SCAN_I];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_I; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int off = block_excl_scan(s, sh, &tot) + tile_sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_I; k++) {
+        if (base + k < n) out[base + k] = off;
+        off += v[k];
+    }
+}
+
+static int* g_tiles       = nullptr;
+static size_t g_tiles_cap = 0;
+
+void exclusive_scan(const int* in, int* out, int n, int* d_total)
+{
+    if (failed()) return;
+    int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntiles < 1) ntiles = 1;
+    if (!grow_n(&g_tiles, &g_tiles_cap, (size_t)ntiles, 0)) return;
+    hipStream_t s = stream();
+    k_scan_reduce<<<ntiles, SCAN_T, 0, s>>>(in, n, g_tiles);
+    k_scan_tiles<<<1, 1024, 0, s>>>(g_tiles, ntiles, d_total);
+    k_scan_apply<<<ntiles, SCAN_T, 0, s>>>(in, out, n, g_tiles);
+    MDG_LAUNCH_CHECK("exclusive_scan");
+}
+
+}  // namespace mdg
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+int mdgfx950Status(void) { return mdg::g_status; }
+const char* mdgfx950StatusMessage(void) { return mdg::g_status ? mdg::g_msg : ""; }
+void mdgfx950ClearStatus(void)
+{
+    mdg::g_status = 0;
+    mdg::g_msg[0] = 0;
+}
+void mdgfx950SetStrict(int strict) { mdg::g_strict = strict != 0; }
+
+void mdgfx950EnableTiming(int on)
+{
+    if (on && !mdg::g_timing) {
+        for (int f = 0; f < mdg::F_NFAMILY; f++) {
+            mdg::g_ms[f]     = 0;
+            mdg::g_launch[f] = 0;
+        }
+    }
+    if (!on) mdg::harvest();
+    mdg::g_timing = on != 0;
+}
+
+int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches)
+{
+    mdg::harvest();
+    for (int f = 0; f < mdg::F_NFAMILY; f++)
+        if (strcmp(family, mdg::g_famname[f]) == 0) {
+            if (total_ms) *total_ms = mdg::g_ms[f];
+            if (launches) *launches = mdg::g_launch[f];
+            return 0;
+        }
+    return -1;
+}
+
+void mdgfx950Synchronize(void)
+{
+    if (mdg::g_stream) MDG_HIP(hipStreamSynchronize(mdg::g_stream));
+}
+
+}  // extern "C"

@@ -28,3 +28,16 @@ def golden():
 def meta(d, key):
     v = d["meta_" + key]
     return v.item()
+
+
+def product():
+    """The product's ctypes binding (md-bench_amd/mdbench_gfx950.py)."""
+    import importlib.util
+    path = os.path.join(REPO, "md-bench_amd", "mdbench_gfx950.py")
+    spec = importlib.util.spec_from_file_location("mdbench_gfx950", path)
+    if "mdbench_gfx950" in sys.modules:
+        return sys.modules["mdbench_gfx950"]
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mdbench_gfx950"] = mod
+    spec.loader.exec_module(mod)
+    return mod

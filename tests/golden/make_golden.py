@@ -30,7 +30,7 @@ DEFS = ("-DATOM_POSITION_AOS -DNBLIST_AOS -DPOSITION_AOS3_SUP -DPRECISION=2 -DLJ
 
 DTYPES = {
     "x": np.float64, "v": np.float64, "f": np.float64, "spline": np.float64,
-    "ksum": np.float64, "epsilon": np.float64, "sigma6": np.float64, "cutforcesq": np.float64,
+    "ksum": np.float64, "bbox": np.float64, "epsilon": np.float64, "sigma6": np.float64, "cutforcesq": np.float64,
     "cutneighsq": np.float64,
 }
 
@@ -91,5 +91,45 @@ def main():
                                          os.path.join(REF, "data", "Cu_u3.eam"), 3.615, 600.0], wd)
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and os.environ.get("MDB_GOLDEN_VL", "1") == "1":
     main()
+
+
+# ---------------------------------------------------------------- cluster pair
+CP_VARIANTS = {
+    # name: (archive, extra defines, march)
+    "m1n2": ("libmdref_cp.a", "-DUSE_REFERENCE_KERNEL -DVECTOR_WIDTH=2", "-march=x86-64-v2"),
+    "m4n4": ("libmdref_cp44.a", "-DVECTOR_WIDTH=4 -D__SIMD_KERNEL__ -D__ISA_AVX2__",
+             "-march=x86-64-v3"),
+    "m4n8": ("libmdref_cp48.a", "-DVECTOR_WIDTH=8 -D__SIMD_KERNEL__ -D__ISA_AVX512__",
+             "-march=x86-64-v4"),
+}
+CP_BASE = ("-DATOM_POSITION_AOS -DNBLIST_AOS -DPOSITION_AOS3_SUP -DPRECISION=2 -DLJ_COMB_RULE=1 "
+           "-DSUPERCLUSTER_INVERSE_THREAD_MAPPING -DALIGNMENT=64 -D_GNU_SOURCE -DNO_ZMM_INTRIN "
+           "-DCLUSTER_PAIR -DCLUSTERPAIR_KERNEL_AUTO").split()
+
+
+def build_cp_dumper(workdir, variant):
+    arch, extra, march = CP_VARIANTS[variant]
+    exe = os.path.join(workdir, "ref_dump_cp_" + variant)
+    cmd = (["gcc", "-O2", "-std=c11", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", march]
+           + CP_BASE + extra.split() +
+           ["-I" + os.path.join(REF, "src", "clusterpair"), "-I" + os.path.join(REF, "src", "common"),
+            os.path.join(HERE, "ref_dump_cp.c"), os.path.join(REFLIB, arch), "-lm", "-o", exe])
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def main_cp():
+    with tempfile.TemporaryDirectory() as wd:
+        for variant in CP_VARIANTS:
+            exe = build_cp_dumper(wd, variant)
+            dump_force = 1 if variant == "m1n2" else 0
+            for half in (0, 1):
+                name = "cp_%s_4%s" % (variant, "_half" if half else "")
+                run_case(exe, name, [4, 4, 4, half, 40, 20, dump_force], wd)
+            run_case(exe, "cp_%s_5x6x7" % variant, [5, 6, 7, 0, 20, 10, dump_force], wd)
+
+
+if __name__ == "__main__" and os.environ.get("MDB_GOLDEN_CP", "1") == "1":
+    main_cp()
