@@ -105,6 +105,30 @@ int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, do
     int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
     int reneigh_every, double mass, double* f, double* ksum);
 
+/* ---------------------------------------------------------------- cluster pair (oracle_cp.c) */
+typedef struct orc_cp orc_cp;
+orc_cp* orc_cp_create(int M, int N, int imask_rule, int fma_dist, double xprd, double yprd,
+    double zprd, double cutneigh, int natoms, const double* x, const double* v, const int* type);
+void orc_cp_free(orc_cp* g);
+int orc_cp_build_clusters(orc_cp* g);
+int orc_cp_define_jclusters(orc_cp* g);
+int orc_cp_setup_pbc(orc_cp* g);
+void orc_cp_update_pbc(orc_cp* g, int first);
+int orc_cp_bin_jclusters(orc_cp* g);
+long long orc_cp_build_neighbor(orc_cp* g, int half);
+int orc_cp_reneighbour(orc_cp* g, int half);
+void orc_cp_force_lj(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq,
+    double* energy, double* virial);
+void orc_cp_initial_integrate(orc_cp* g, double dt, double dtforce);
+void orc_cp_final_integrate(orc_cp* g, double dtforce);
+void orc_cp_update_atoms(orc_cp* g);
+double orc_cp_kinetic_sum(const orc_cp* g, double mass);
+int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq, double dt,
+    double dtforce, int ntimes, int reneigh_every, double mass, double* ksum);
+void orc_cp_info(const orc_cp* g, int* info, long long* nb_total);
+long long orc_cp_get(const orc_cp* g, const char* name, void* out, long long cap);
+void orc_cp_set_slot_x(orc_cp* g, const double* x, int nslots);
+
 #ifdef __cplusplus
 }
 #endif

@@ -270,6 +270,20 @@ int main(int argc, char** argv)
         } else {
             updateSingleAtoms(&param, &atom);
             updateAtomsPbc(&atom, &param, false);
+            if (n + 1 == param.reneigh_every) {
+                /* atoms handed to the first rebuild (cluster order, wrapped) */
+                for (int i = 0; i < atom.Nlocal; i++) {
+                    ax[3 * i]     = atom.x[3 * i];
+                    ax[3 * i + 1] = atom.x[3 * i + 1];
+                    ax[3 * i + 2] = atom.x[3 * i + 2];
+                    av[3 * i]     = atom.vx[i];
+                    av[3 * i + 1] = atom.vy[i];
+                    av[3 * i + 2] = atom.vz[i];
+                }
+                dump("pre1_x", ax, sizeof(double) * 3 * atom.Nlocal);
+                dump("pre1_v", av, sizeof(double) * 3 * atom.Nlocal);
+                dump("pre1_type", atom.type, sizeof(int) * atom.Nlocal);
+            }
             buildClusters(&atom);
             defineJClusters(&param, &atom);
             setupPbc(&atom, &param);

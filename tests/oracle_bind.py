@@ -191,3 +191,95 @@ def vl_run(nx, ny, nz, lattice, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutn
                           dt, dtforce, ntimes, reneigh, mass, f, ks)
     assert rc == 0, rc
     return xx, vv, f, ks
+
+
+# ---------------------------------------------------------------- cluster pair
+CP_RULES = {(1, 2): (2, 0), (4, 4): (4, 1), (4, 8): (8, 1)}  # (imask rule, fma in list test)
+
+
+class CP:
+    """Cluster-pair oracle context (oracle/oracle_cp.c)."""
+
+    INFO = ["M", "N", "nci", "ncj", "nghost", "dummy_cj", "mbins", "nstencil", "nbinx", "nbiny",
+            "mbinx", "mbiny", "mbinxlo", "mbinylo", "natoms"]
+
+    def __init__(self, M, N, prd, cutneigh, x, v, t, fma_dist=None):
+        L = lib()
+        if not getattr(L, "_cp_bound", False):
+            L.orc_cp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                        _ip]
+            L.orc_cp_create.restype = C.c_void_p
+            for fn in ("orc_cp_free", "orc_cp_build_clusters", "orc_cp_define_jclusters",
+                       "orc_cp_setup_pbc", "orc_cp_bin_jclusters", "orc_cp_update_atoms"):
+                getattr(L, fn).argtypes = [C.c_void_p]
+            L.orc_cp_update_pbc.argtypes = [C.c_void_p, C.c_int]
+            L.orc_cp_build_neighbor.argtypes = [C.c_void_p, C.c_int]
+            L.orc_cp_build_neighbor.restype = C.c_longlong
+            L.orc_cp_reneighbour.argtypes = [C.c_void_p, C.c_int]
+            L.orc_cp_force_lj.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+            L.orc_cp_initial_integrate.argtypes = [C.c_void_p, C.c_double, C.c_double]
+            L.orc_cp_final_integrate.argtypes = [C.c_void_p, C.c_double]
+            L.orc_cp_kinetic_sum.argtypes = [C.c_void_p, C.c_double]
+            L.orc_cp_kinetic_sum.restype = C.c_double
+            L.orc_cp_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _dp]
+            L.orc_cp_info.argtypes = [C.c_void_p, _ip, C.POINTER(C.c_longlong)]
+            L.orc_cp_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong]
+            L.orc_cp_get.restype = C.c_longlong
+            L.orc_cp_set_slot_x.argtypes = [C.c_void_p, _dp, C.c_int]
+            L._cp_bound = True
+        rule, fm = CP_RULES[(M, N)]
+        if fma_dist is not None:
+            fm = fma_dist
+        self.L = L
+        self.h = L.orc_cp_create(M, N, rule, fm, prd[0], prd[1], prd[2], cutneigh,
+                                 x.size // 3, np.ascontiguousarray(x, np.float64),
+                                 np.ascontiguousarray(v, np.float64),
+                                 np.ascontiguousarray(t, np.int32))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_cp_free(self.h)
+            self.h = None
+
+    def info(self):
+        arr = np.zeros(15, np.int32)
+        tot = C.c_longlong()
+        self.L.orc_cp_info(self.h, arr, C.byref(tot))
+        d = dict(zip(self.INFO, arr.tolist()))
+        d["nb_total"] = tot.value
+        return d
+
+    def get(self, name):
+        dt = np.float64 if name in ("slot_x", "slot_v", "slot_f", "ci_bbox", "cj_bbox",
+                                    "atoms_x", "atoms_v") else np.int32
+        if name == "imask":
+            dt = np.uint32
+        n = self.L.orc_cp_get(self.h, name.encode(), None, 0)
+        n = -n if n < 0 else n
+        # ask for the size: call with cap 0 returns -2 when non-empty
+        cap = 1 << 26
+        buf = np.zeros(cap if dt != np.float64 else cap // 2, dt)
+        k = self.L.orc_cp_get(self.h, name.encode(), buf.ctypes.data_as(C.c_void_p), buf.size)
+        assert k >= 0, (name, k)
+        return buf[:k].copy()
+
+    def reneighbour(self, half):
+        rc = self.L.orc_cp_reneighbour(self.h, int(half))
+        assert rc == 0, rc
+
+    def force(self, half, eps=1.0, sig6=1.0, cutf=6.25):
+        e, w = C.c_double(), C.c_double()
+        self.L.orc_cp_force_lj(self.h, int(half), eps, sig6, cutf, C.byref(e), C.byref(w))
+        return self.get("slot_f"), e.value, w.value
+
+    def run(self, half, ntimes, reneigh, dt=0.005, dtforce=0.0025, mass=1.0, eps=1.0, sig6=1.0,
+            cutf=6.25):
+        ks = np.zeros(ntimes + 1)
+        rc = self.L.orc_cp_run(self.h, int(half), eps, sig6, cutf, dt, dtforce, ntimes, reneigh,
+                               mass, ks)
+        assert rc == 0, rc
+        return ks
