@@ -8,8 +8,11 @@ ghost update, force, integrate; a neighbour rebuild every reneigh_every steps), 
 already resident in HBM.  `value` = atoms x steps / wall time of the K timed steps.
 
 Workloads (seeded fcc lattices, synthetic velocities):
-  vl-lj-64   Verlet-list LJ, FP64, full list, 64^3 cells = 1,048,576 atoms, rho* 0.8442,
-             r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
+  cp-lj-64   (default) cluster-pair LJ, i-clusters of 4 against j-clusters of 4, FP32 pair
+             math with FP64 positions/velocities, full list, 64^3 cells = 1,048,576 atoms,
+             rho* 0.8442, r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
+  cp-lj-64-4x8, cp-lj-64-half: the same with j-clusters of 8 / a half list.
+  vl-lj-64   Verlet-list LJ, FP64, full list, same lattice.
 
 `roofline` covers the dominant kernel (the force kernel): algorithmic bytes per launch over its
 average launch time, both measured inside the timed region with HIP events on the library stream.
@@ -130,8 +133,82 @@ def bench_vl(args, mod):
         sim.close()
 
 
-WORKLOADS = {"vl-lj-64": bench_vl}
-DEFAULT = "vl-lj-64"
+def cpu_baseline_cp(nx, steps):
+    cores = host_cores()
+    flags = cpu_flags()
+    name = "mdbench-cp-avx512" if "avx512f" in flags else "mdbench-cp-avx2"
+    ref = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(ref):
+        return None
+    v = run_reference(ref, ["-nx", str(nx), "-ny", str(nx), "-nz", str(nx), "-n", str(steps)],
+                      cores)
+    return {"value": v, "unit": "atom-steps/s", "cores": cores, "kind": "reference",
+            "sample": "%d^3 cells (%d atoms), %d timesteps, MD-Bench cluster-pair 4xN SIMD "
+                      "(%s, -Ofast, 'DP' with approximate reciprocal), full list"
+                      % (nx, 4 * nx ** 3, steps, name.split("-")[-1])}
+
+
+def bench_cp(args, mod, N=4, half=0):
+    import ctypes as C
+    nx = 64
+    sim = mod.CPSim(nx=nx, ny=nx, nz=nx, ntimes=args.steps + args.warmup, cluster_m=4,
+                    cluster_n=N, half_neigh=half, precision=0)
+    try:
+        natoms = 4 * nx ** 3
+        sim.run(args.warmup)
+        L = mod.lib()
+        L.mdgfx950Synchronize()
+        L.mdgfx950EnableTiming(1)
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        L.mdgfx950Synchronize()
+        t1 = time.perf_counter()
+        ms = C.c_double()
+        nl = C.c_longlong()
+        L.mdgfx950KernelTime(b"force_cp", C.byref(ms), C.byref(nl))
+        L.mdgfx950EnableTiming(0)
+        info = sim.info()
+        pairs = int(sim.get("numneigh").astype(np.int64).sum())
+        nci = info["nci"]
+        elapsed = t1 - t0
+        # Algorithmic bytes of one FP32-mixed cluster-pair force launch: per i-cluster the list
+        # count and cluster size (8 B), 4 FP64 positions (96 B) and 4 FP32 forces (48 B); per
+        # cluster pair the j-cluster index (4 B) and N FP64 positions (24 N B).
+        bytes_launch = nci * (8.0 + 96.0 + 48.0) + pairs * (4.0 + 24.0 * N)
+        avg_ms = ms.value / max(nl.value, 1)
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        wl = "cp-lj-64%s%s" % ("-4x8" if N == 8 else "", "-half" if half else "")
+        res = {
+            "metric": METRIC, "value": natoms * args.steps / elapsed, "unit": "atom-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair math, f64 state",
+            "data": "synthetic",
+            "config": {"workload": "%s: LJ cluster-pair 4x%d %s list, FP32-mixed, 64^3 fcc "
+                                   "cells (%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20"
+                                   % (wl, N, "half" if half else "full", natoms),
+                       "atoms": natoms, "i_clusters": nci,
+                       "j_clusters_per_i_cluster": pairs / nci},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_cp_force_f32 (one wavefront per i-cluster)",
+                         "avg_launch_ms": avg_ms, "launches": nl.value,
+                         "bytes_per_launch": bytes_launch},
+        }
+        return res, nx
+    finally:
+        sim.close()
+
+
+import numpy as np  # noqa: E402
+
+WORKLOADS = {
+    "cp-lj-64": bench_cp,
+    "cp-lj-64-4x8": lambda a, m: bench_cp(a, m, N=8),
+    "cp-lj-64-half": lambda a, m: bench_cp(a, m, half=1),
+    "vl-lj-64": bench_vl,
+}
+DEFAULT = "cp-lj-64"
 
 
 def main():
@@ -151,7 +228,7 @@ def main():
     cb = None
     if not args.no_cpu_baseline:
         try:
-            cb = cpu_baseline_vl(nx, 20)
+            cb = (cpu_baseline_vl if args.workload.startswith("vl") else cpu_baseline_cp)(nx, 20)
         except Exception as exc:  # the GPU figure stands on its own
             print("cpu baseline failed: %s" % exc, file=sys.stderr)
     res["cpu_baseline"] = cb

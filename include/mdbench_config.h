@@ -1,0 +1,57 @@
+/*
+ * Run configuration and host-side setup helpers shared by the Verlet-list and cluster-pair
+ * drivers (plain C types only, so either scheme's structs can be included next to it).
+ */
+#ifndef MDBENCH_CONFIG_H
+#define MDBENCH_CONFIG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nz;
+    int force_field;   /* FF_LJ or FF_EAM */
+    int half_neigh;
+    int ntimes, nstat, reneigh_every;
+    double rho;        /* > 0 overrides the reduced density */
+    double lattice_a;  /* > 0 overrides the fcc lattice constant (rho = 4 / a^3) */
+    double temp;       /* > 0 overrides the initial temperature */
+    double dt;         /* > 0 overrides the time step */
+    double cutforce;   /* > 0 overrides the force cutoff */
+    double skin;       /* >= 0 overrides the skin (use -1 for the default) */
+    int ntypes;        /* > 0 overrides the number of types */
+    const double* epsilon_per_type; /* optional, ntypes values (geometric mixing) */
+    const double* sigma_per_type;
+    const char* eam_file; /* FF_EAM: funcfl file; NULL = synthetic Cu-like tables */
+    int strict;        /* 1 = strict kernels (bit-identical FP64) */
+    /* cluster-pair scheme only */
+    int cluster_m, cluster_n; /* i- and j-cluster sizes: 4x4 (default), 4x8, 1x2 */
+    int precision;     /* 0 = FP32 pair math with FP64 state (default), 1 = FP64 */
+    int compensated;   /* 1 = Kahan-compensated FP32 force sums */
+} mdb_config;
+
+/* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */
+void mdbDefaultConfig(mdb_config* cfg);
+
+/* Host setup pieces, exported for tests (no device work). */
+/* Parameters, fcc lattice and thermo adjustment of setup() only: fills x, v (3N) and type (N)
+   of up to cap atoms; returns the atom count or a negative error. */
+int mdbCreateLattice(const mdb_config* cfg, double* x, double* v, int* type, int cap);
+double mdbMyrandom(int* seed); /* Park-Miller, src/common/util.c:24-33 */
+/* funcfl tables -> 7-coefficient splines (src/common/eam_utils.c:42-284), one element. */
+typedef struct {
+    int nr, nrho, nr_tot, nrho_tot;
+    double dr, drho, rdr, rdrho, cut, mass;
+    double *rhor_spline, *frho_spline, *z2r_spline;
+} mdb_eam_tables;
+int mdbEamLoad(const char* path, mdb_eam_tables* t);
+void mdbEamFree(mdb_eam_tables* t);
+/* Writes the synthetic Cu-like funcfl file (rho = exp(-3(r/2.556-1)) with a smooth cutoff,
+   Morse-like Z, F = -sqrt(rho); nr = nrho = 2000, cutoff 4.95). */
+int mdbWriteFuncflSynthetic(const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1595 @@
+// Cluster-pair (GROMACS NBNXM-style) scheme on gfx950.
+//
+// Rebuild chain (every reneigh_every steps), all on the device and reproducing the reference's
+// structures exactly: updateSingleAtoms -> wrap -> 2D binning (stable counting sort + the
+// reference's selection sort by z, emulated one wave per bin) -> i-clusters of M -> j-clusters
+// of N -> ghost j-clusters -> j-cluster bins (stable by bbminz) -> cluster-pair lists (one wave
+// per i-cluster, ordered compaction with ballots, masked entries first).
+//
+// Force kernels:
+//   k_cp_force_f64  one thread per i-slot, the reference kernel's arithmetic (computeForceLJRef,
+//                   src/clusterpair/force_lj.c:22-189); full lists are bit-identical to it.
+//   k_cp_force_f32  one wavefront per i-cluster of 4 atoms: lane = (i-atom 0..3) x (j-slot
+//                   0..15), i.e. four 4-atom or two 8-atom j-clusters per iteration.  The pair
+//                   displacement is formed in FP64 from the FP64 positions and rounded once to
+//                   FP32; everything after runs in FP32 with a Newton-refined v_rcp_f32.  The
+//                   full list is atomic-free; the half list reduces j-forces over the four
+//                   i-lanes with cross-lane shuffles and issues one atomic per j-atom.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "mdgfx950_cp.h"
+#include "runtime.h"
+
+using namespace mdg;
+
+namespace {
+
+constexpr int BLOCK       = 256;
+constexpr int MAXT2       = 64;   // ntypes <= 8
+constexpr int BIN_CAP     = 2048; // atoms per 2D column handled in LDS
+constexpr int JBIN_CAP    = 1024; // j-clusters per column handled in LDS
+constexpr double PAD_BASE = 1.0e12, PAD_STEP = 1.0e3;
+constexpr unsigned IMASK_ALL = 0xffffffffu;
+
+enum { ST_BIN_RANGE = 1, ST_NB_OVERFLOW = 2, ST_BIN_CAP = 4, ST_JBIN_CAP = 8 };
+
+struct CGrid {
+    double xprd, yprd, zprd;
+    double bininvx, bininvy;
+    int nbinx, nbiny, mbinxlo, mbinylo, mbinx, mbiny, mbins;
+};
+
+struct Tab64 {
+    double eps[MAXT2], sig6[MAXT2], cutf[MAXT2];
+};
+struct Tab32 {
+    float eps48[MAXT2], eps4[MAXT2], sig6[MAXT2], cut2[MAXT2];
+};
+
+struct CPState {
+    bool ready = false;
+    int M = 4, N = 4, rule = 4, fma_dist = 1;
+    int precision = 0, comp = 0;
+    CGrid g{};
+    double binsizex = 0, binsizey = 0, cutneigh = 0, cutneighsq = 0, rbb_sq = 0, cutforce = 0;
+    int nstencil = 0;
+    int* d_stencil = nullptr;
+    size_t cap_stencil = 0;
+    int natoms = 0;
+    // atoms (reference order)
+    double *d_ax = nullptr, *d_av = nullptr;
+    int* d_at  = nullptr;
+    size_t cap_ax = 0, cap_av = 0, cap_at = 0;
+    int *d_abin_of = nullptr, *d_abin_cnt = nullptr, *d_abin_start = nullptr,
+        *d_abin_atoms = nullptr;
+    size_t cap_abin_of = 0, cap_abin_cnt = 0, cap_abin_start = 0, cap_abin_atoms = 0;
+    // clusters
+    int nci = 0, ncj = 0, nghost = 0, dummy_cj = 0, nghost_atoms = 0;
+    int *d_ncl = nullptr, *d_cstart = nullptr;
+    size_t cap_ncl = 0, cap_cstart = 0;
+    int *d_ci_natoms = nullptr, *d_ci_bin = nullptr, *d_cj_natoms = nullptr;
+    double *d_ci_bbox = nullptr, *d_cj_bbox = nullptr;
+    size_t cap_ci_natoms = 0, cap_ci_bin = 0, cap_cj_natoms = 0, cap_ci_bbox = 0, cap_cj_bbox = 0;
+    double *d_sx = nullptr, *d_sv = nullptr, *d_sf64 = nullptr;
+    float* d_sf32 = nullptr;
+    int* d_st     = nullptr;
+    size_t cap_sx = 0, cap_sv = 0, cap_sf64 = 0, cap_sf32 = 0, cap_st = 0;
+    int *d_aoff = nullptr;
+    size_t cap_aoff = 0;
+    // ghosts
+    int *d_gcnt = nullptr, *d_goff = nullptr, *d_border = nullptr, *d_pbc = nullptr,
+        *d_gbin = nullptr;
+    size_t cap_gcnt = 0, cap_goff = 0, cap_border = 0, cap_pbc = 0, cap_gbin = 0;
+    // j-cluster bins
+    int *d_jcnt = nullptr, *d_jstart = nullptr, *d_jlist = nullptr, *d_gcur = nullptr;
+    size_t cap_jcnt = 0, cap_jstart = 0, cap_jlist = 0, cap_gcur = 0;
+    int *d_gbin_start = nullptr, *d_gbin_list = nullptr;
+    size_t cap_gbin_start = 0, cap_gbin_list = 0;
+    // lists
+    int maxneighs = 200;
+    int *d_nb = nullptr, *d_numneigh = nullptr, *d_nmasked = nullptr;
+    size_t cap_nb = 0, cap_numneigh = 0, cap_nmasked = 0;
+    // parameters
+    int ntypes = 1;
+    bool uniform = true;
+    Tab64 t64{};
+    Tab32 t32{};
+    double dt = 0, dtforce = 0, mass = 1.0;
+    bool forces_are_f32 = false;
+    // reductions
+    double *d_red = nullptr, *d_ev = nullptr;
+    size_t cap_red = 0;
+    int* d_status = nullptr;
+    int* h_status = nullptr;
+    double* h_red = nullptr;
+    bool ev_on    = false;
+};
+
+CPState C;
+
+// ---------------------------------------------------------------- device helpers
+__device__ inline void c2b2d(const CGrid& g, double x, double y, int* ix, int* iy)
+{
+    if (x >= g.xprd) *ix = (int)((x - g.xprd) * g.bininvx) + g.nbinx - g.mbinxlo;
+    else if (x >= 0.0) *ix = (int)(x * g.bininvx) - g.mbinxlo;
+    else *ix = (int)(x * g.bininvx) - g.mbinxlo - 1;
+    if (y >= g.yprd) *iy = (int)((y - g.yprd) * g.bininvy) + g.nbiny - g.mbinylo;
+    else if (y >= 0.0) *iy = (int)(y * g.bininvy) - g.mbinylo;
+    else *iy = (int)(y * g.bininvy) - g.mbinylo - 1;
+}
+
+__device__ inline unsigned imask_of(int rule, int ci, int cj)
+{
+    // get_imask_simd_j2/j4/j8 with rdiag = 1 (src/clusterpair/neighbor.c:240-280)
+    if (rule == 2) return (ci * 2 == cj) ? 0x0002u : ((ci * 2 + 1 == cj) ? 0x002fu : IMASK_ALL);
+    if (rule == 4) return (ci == cj) ? 0x08ceu : IMASK_ALL;
+    return (ci == cj * 2) ? 0xf0f8fcfeu : ((ci == cj * 2 + 1) ? 0x0080c0e0u : IMASK_ALL);
+}
+
+__device__ inline double pad_pos(int slot, int d) { return PAD_BASE + (double)slot * PAD_STEP + d; }
+
+// 26 ghost images in the reference's order (src/clusterpair/pbc.c:248-317): side per axis,
+// 1 = low face (shift +1), 2 = high face (shift -1).
+__constant__ signed char c_side[26][3] = { { 1, 0, 0 }, { 2, 0, 0 }, { 0, 1, 0 }, { 0, 2, 0 },
+    { 0, 0, 1 }, { 0, 0, 2 }, { 1, 1, 1 }, { 1, 2, 1 }, { 1, 1, 2 }, { 1, 2, 2 }, { 2, 1, 1 },
+    { 2, 2, 1 }, { 2, 1, 2 }, { 2, 2, 2 }, { 1, 0, 1 }, { 1, 0, 2 }, { 2, 0, 1 }, { 2, 0, 2 },
+    { 0, 1, 1 }, { 0, 1, 2 }, { 0, 2, 1 }, { 0, 2, 2 }, { 1, 1, 0 }, { 2, 1, 0 }, { 1, 2, 0 },
+    { 2, 2, 0 } };
+
+__device__ inline unsigned ghost_mask_bb(const double* bb, double xprd, double yprd, double zprd,
+    double cut)
+{
+    const double prd[3] = { xprd, yprd, zprd };
+    bool lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = bb[2 * d] < cut;
+        hi[d] = bb[2 * d + 1] >= (prd[d] - cut);
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int c = 0; c < 26; c++) {
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (c_side[c][d] == 1) ok = ok && lo[d];
+            if (c_side[c][d] == 2) ok = ok && hi[d];
+        }
+        if (ok) m |= 1u << c;
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------- updateSingleAtoms + wrap
+__global__ void k_cp_slots_to_atoms(int nci, int M, const int* __restrict__ ci_natoms,
+    const int* __restrict__ aoff, const double* __restrict__ sx, const double* __restrict__ sv,
+    const int* __restrict__ st, double* __restrict__ ax, double* __restrict__ av,
+    int* __restrict__ at)
+{
+    int s  = blockIdx.x * blockDim.x + threadIdx.x;
+    int ci = s / M, cii = s % M;
+    if (ci >= nci || cii >= ci_natoms[ci]) return;
+    int a = aoff[ci] + cii;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ax[3 * a + d] = sx[3 * s + d];
+        av[3 * a + d] = sv[3 * s + d];
+    }
+    at[a] = st[s];
+}
+
+__global__ void k_cp_wrap(int n3, double* __restrict__ x, double xprd, double yprd, double zprd)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    int d      = i % 3;
+    double prd = d == 0 ? xprd : (d == 1 ? yprd : zprd);
+    double xx  = x[i];
+    if (xx < 0.0) xx += prd;
+    else if (xx >= prd) xx -= prd;
+    x[i] = xx;
+}
+
+// ---------------------------------------------------------------- binAtoms (neighbor.c:1409-1449)
+__global__ void k_cp_bin_count(int natoms, const double* __restrict__ ax, CGrid g,
+    int* __restrict__ bin_of, int* __restrict__ cnt, int* __restrict__ status)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    int ix, iy;
+    c2b2d(g, ax[3 * i], ax[3 * i + 1], &ix, &iy);
+    int b = iy * g.mbinx + ix + 1;
+    if (b < 0 || b >= g.mbins) {
+        atomicOr(status, ST_BIN_RANGE);
+        bin_of[i] = -1;
+        return;
+    }
+    bin_of[i] = b;
+    atomicAdd(&cnt[b], 1);
+}
+
+__global__ void k_cp_bin_scatter(int natoms, const int* __restrict__ bin_of,
+    const int* __restrict__ start, int* __restrict__ cursor, int* __restrict__ atoms)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    int b = bin_of[i];
+    if (b < 0) return;
+    atoms[start[b] + atomicAdd(&cursor[b], 1)] = i;
+}
+
+// One wavefront per bin: restore atom-index order (the reference fills bins in index order),
+// then replay sortAtomsByZCoord (neighbor.c:1452-1481): at step a, the first position holding
+// the minimum z among positions a..c-1 swaps with position a.  Also counts the bin's clusters.
+__global__ void __launch_bounds__(64) k_cp_bin_sort(const int* __restrict__ start,
+    int* __restrict__ atoms, const double* __restrict__ ax, int M, int N, int* __restrict__ ncl,
+    int* __restrict__ status)
+{
+    __shared__ int s_idx[BIN_CAP];
+    __shared__ int s_tmp[BIN_CAP];
+    __shared__ double s_z[BIN_CAP];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int s0 = start[b], c = start[b + 1] - s0;
+    if (c > BIN_CAP) {
+        if (lane == 0) {
+            atomicOr(status, ST_BIN_CAP);
+            ncl[b] = 0;
+        }
+        return;
+    }
+    for (int p = lane; p < c; p += 64)
+        s_idx[p] = atoms[s0 + p];
+    __syncthreads();
+    for (int p = lane; p < c; p += 64) {
+        int v = s_idx[p], r = 0;
+        for (int q = 0; q < c; q++)
+            r += s_idx[q] < v;
+        s_tmp[r] = v;
+    }
+    __syncthreads();
+    for (int p = lane; p < c; p += 64) {
+        s_idx[p] = s_tmp[p];
+        s_z[p]   = ax[3 * s_tmp[p] + 2];
+    }
+    __syncthreads();
+    for (int a = 0; a < c; a++) {
+        double bz = INFINITY;
+        int bp    = INT_MAX;
+        for (int p = a + lane; p < c; p += 64) {
+            double zz = s_z[p];
+            if (zz < bz || (zz == bz && p < bp)) {
+                bz = zz;
+                bp = p;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            double oz = __shfl_xor(bz, o, 64);
+            int op    = __shfl_xor(bp, o, 64);
+            if (oz < bz || (oz == bz && op < bp)) {
+                bz = oz;
+                bp = op;
+            }
+        }
+        if (lane == 0) {
+            int mp = (bp == INT_MAX) ? a : bp;
+            int ti = s_idx[a];
+            double tz = s_z[a];
+            s_idx[a]  = s_idx[mp];
+            s_z[a]    = s_z[mp];
+            s_idx[mp] = ti;
+            s_z[mp]   = tz;
+        }
+        __syncthreads();
+    }
+    for (int p = lane; p < c; p += 64)
+        atoms[s0 + p] = s_idx[p];
+    if (lane == 0) {
+        int nc = (c + M - 1) / M;
+        if (N > M && (nc % 2)) nc++;
+        ncl[b] = nc;
+    }
+}
+
+__global__ void k_cp_fill_ci_bin(int mbins, const int* __restrict__ cstart, const int* __restrict__ ncl,
+    int* __restrict__ ci_bin)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= mbins) return;
+    for (int k = 0; k < ncl[b]; k++)
+        ci_bin[cstart[b] + k] = b;
+}
+
+// buildClustersCPU (neighbor.c:1513-1611): cluster ci holds atoms cl*M .. cl*M+M-1 of its bin.
+__global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bin,
+    const int* __restrict__ cstart, const int* __restrict__ abin_start,
+    const int* __restrict__ abin_atoms, const double* __restrict__ ax,
+    const double* __restrict__ av, const int* __restrict__ at, double* __restrict__ sx,
+    double* __restrict__ sv, int* __restrict__ st, int* __restrict__ ci_natoms,
+    double* __restrict__ ci_bbox)
+{
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nci) return;
+    int b  = ci_bin[ci];
+    int cl = ci - cstart[b];
+    int s0 = abin_start[b], c = abin_start[b + 1] - s0;
+    double bb[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+    int na = 0;
+    for (int cii = 0; cii < M; cii++) {
+        int ac = cl * M + cii;
+        int s  = ci * M + cii;
+        if (ac < c) {
+            int i = abin_atoms[s0 + ac];
+            for (int d = 0; d < 3; d++) {
+                double xv    = ax[3 * i + d];
+                sx[3 * s + d] = xv;
+                sv[3 * s + d] = av[3 * i + d];
+                if (bb[2 * d] > xv) bb[2 * d] = xv;
+                if (bb[2 * d + 1] < xv) bb[2 * d + 1] = xv;
+            }
+            st[s] = at[i];
+            na++;
+        } else {
+            for (int d = 0; d < 3; d++) {
+                sx[3 * s + d] = pad_pos(s, d);
+                sv[3 * s + d] = 0.0;
+            }
+            st[s] = 0;
+        }
+    }
+    ci_natoms[ci] = na;
+    for (int k = 0; k < 6; k++)
+        ci_bbox[6 * ci + k] = bb[k];
+}
+
+// defineJClusters (neighbor.c:1807-1927), shapes M == N and N == 2M.
+__global__ void k_cp_define_j(int ncj, int M, int N, const int* __restrict__ ci_natoms,
+    const double* __restrict__ ci_bbox, int* __restrict__ cj_natoms, double* __restrict__ cj_bbox)
+{
+    int cj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cj >= ncj) return;
+    if (M == N) {
+        cj_natoms[cj] = ci_natoms[cj];
+        for (int k = 0; k < 6; k++)
+            cj_bbox[6 * cj + k] = ci_bbox[6 * cj + k];
+    } else {
+        int c0 = 2 * cj, c1 = 2 * cj + 1;
+        for (int d = 0; d < 3; d++) {
+            cj_bbox[6 * cj + 2 * d]     = fmin(ci_bbox[6 * c0 + 2 * d], ci_bbox[6 * c1 + 2 * d]);
+            cj_bbox[6 * cj + 2 * d + 1] = fmax(ci_bbox[6 * c0 + 2 * d + 1], ci_bbox[6 * c1 + 2 * d + 1]);
+        }
+        cj_natoms[cj] = ci_natoms[c0] + ci_natoms[c1];
+    }
+}
+
+// ---------------------------------------------------------------- setupPbc (pbc.c:188-331)
+__global__ void k_cp_ghost_count(int ncj, const int* __restrict__ cj_natoms,
+    const double* __restrict__ cj_bbox, double xprd, double yprd, double zprd, double cut,
+    int* __restrict__ cnt)
+{
+    int cj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cj >= ncj) return;
+    cnt[cj] = cj_natoms[cj] > 0 ? __popc(ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut)) : 0;
+}
+
+__global__ void k_cp_ghost_fill(int ncj, int N, const int* __restrict__ cj_natoms_in,
+    const double* __restrict__ cj_bbox, double xprd, double yprd, double zprd, double cut,
+    const int* __restrict__ off, int* __restrict__ border, int* __restrict__ pbc,
+    int* __restrict__ cj_natoms, int* __restrict__ st, int* __restrict__ status)
+{
+    int cj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cj >= ncj) return;
+    int na = cj_natoms_in[cj];
+    if (na <= 0) return;
+    unsigned m = ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut);
+    int g      = off[cj];
+    for (int c = 0; c < 26; c++) {
+        if (!(m & (1u << c))) continue;
+        border[g] = cj;
+        for (int d = 0; d < 3; d++)
+            pbc[3 * g + d] = c_side[c][d] == 1 ? 1 : (c_side[c][d] == 2 ? -1 : 0);
+        int cg        = ncj + g;
+        cj_natoms[cg] = na;
+        for (int cjj = 0; cjj < na; cjj++)
+            st[cg * N + cjj] = st[cj * N + cjj];
+        atomicAdd(&status[3], na);
+        g++;
+    }
+}
+
+// updatePbcCPU with firstUpdate = 1: positions, padding and bounding boxes of ghost clusters.
+__global__ void k_cp_ghost_first(int ncj, int nghost, int N, const int* __restrict__ border,
+    const int* __restrict__ pbc, const int* __restrict__ cj_natoms, double xprd, double yprd,
+    double zprd, double* __restrict__ sx, int* __restrict__ st, double* __restrict__ cj_bbox)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > nghost) return;
+    int cg = ncj + g;
+    if (g == nghost) {  // the dummy cluster
+        for (int cjj = 0; cjj < N; cjj++) {
+            int s = cg * N + cjj;
+            for (int d = 0; d < 3; d++)
+                sx[3 * s + d] = pad_pos(s, d);
+            st[s] = 0;
+        }
+        return;
+    }
+    const double prd[3] = { xprd, yprd, zprd };
+    int b  = border[g];
+    int na = cj_natoms[cg];
+    double bb[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+    for (int cjj = 0; cjj < N; cjj++) {
+        int s = cg * N + cjj;
+        if (cjj < na) {
+            int sb = b * N + cjj;
+            for (int d = 0; d < 3; d++) {
+                double xv    = sx[3 * sb + d] + pbc[3 * g + d] * prd[d];
+                sx[3 * s + d] = xv;
+                if (bb[2 * d] > xv) bb[2 * d] = xv;
+                if (bb[2 * d + 1] < xv) bb[2 * d + 1] = xv;
+            }
+        } else {
+            for (int d = 0; d < 3; d++)
+                sx[3 * s + d] = pad_pos(s, d);
+            st[s] = 0;
+        }
+    }
+    for (int k = 0; k < 6; k++)
+        cj_bbox[6 * cg + k] = bb[k];
+}
+
+// updatePbcCPU with firstUpdate = 0: one thread per ghost slot coordinate.
+__global__ void k_cp_ghost_update(int ncj, int nghost, int N, const int* __restrict__ border,
+    const int* __restrict__ pbc, const int* __restrict__ cj_natoms, double xprd, double yprd,
+    double zprd, double* __restrict__ sx)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * nghost * N) return;
+    int d = k % 3, s = k / 3;
+    int g = s / N, cjj = s % N;
+    if (cjj >= cj_natoms[ncj + g]) return;
+    double prd = d == 0 ? xprd : (d == 1 ? yprd : zprd);
+    int sb     = border[g] * N + cjj;
+    sx[3 * ((ncj + g) * N + cjj) + d] = sx[3 * sb + d] + pbc[3 * g + d] * prd;
+}
+
+// ---------------------------------------------------------------- binJClusters (neighbor.c:1929-2094)
+__global__ void k_cp_ghost_bin(int ncj, int nghost, int N, CGrid g, const int* __restrict__ cj_natoms,
+    const double* __restrict__ sx, const int* __restrict__ pbc, int* __restrict__ gbin,
+    int* __restrict__ gcnt, int* __restrict__ status)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nghost) return;
+    int cj = ncj + k, na = cj_natoms[cj];
+    gbin[k] = -1;
+    if (na <= 0) return;
+    int ix, iy;
+    c2b2d(g, sx[3 * cj * N], sx[3 * cj * N + 1], &ix, &iy);
+    ix = max(min(ix, g.mbinx - 1), 0);
+    iy = max(min(iy, g.mbiny - 1), 0);
+    for (int cjj = 1; cjj < na; cjj++) {
+        int nix, niy;
+        int s = cj * N + cjj;
+        c2b2d(g, sx[3 * s], sx[3 * s + 1], &nix, &niy);
+        nix = max(min(nix, g.mbinx - 1), 0);
+        niy = max(min(niy, g.mbiny - 1), 0);
+        if (pbc[3 * k] > 0 && ix > nix) ix = nix;
+        if (pbc[3 * k] < 0 && ix < nix) ix = nix;
+        if (pbc[3 * k + 1] > 0 && iy > niy) iy = niy;
+        if (pbc[3 * k + 1] < 0 && iy < niy) iy = niy;
+    }
+    int b = iy * g.mbinx + ix + 1;
+    if (b < 0 || b >= g.mbins) {
+        atomicOr(status, ST_BIN_RANGE);
+        return;
+    }
+    gbin[k] = b;
+    atomicAdd(&gcnt[b], 1);
+}
+
+__global__ void k_cp_jbin_count(int mbins, int M, int N, const int* __restrict__ ncl,
+    const int* __restrict__ gcnt, int* __restrict__ jcnt)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= mbins) return;
+    jcnt[b] = (M == N ? ncl[b] : ncl[b] / 2) + gcnt[b];
+}
+
+__global__ void k_cp_ghost_scatter(int nghost, const int* __restrict__ gbin,
+    const int* __restrict__ gstart, int* __restrict__ gcur, int* __restrict__ glist)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nghost) return;
+    int b = gbin[k];
+    if (b < 0) return;
+    glist[gstart[b] + atomicAdd(&gcur[b], 1)] = k;
+}
+
+// One wavefront per bin: locals (in cluster order) then ghosts, stably ordered by bbminz --
+// the order the reference's sorted insertion of ghosts produces.  Ties go to the smaller
+// j-cluster index (locals before ghosts, ghosts in creation order).
+__global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
+    const int* __restrict__ ncl, const int* __restrict__ cstart, const int* __restrict__ gstart,
+    const int* __restrict__ glist, const double* __restrict__ cj_bbox,
+    const int* __restrict__ jstart, int* __restrict__ jlist, int* __restrict__ status)
+{
+    __shared__ int s_cj[JBIN_CAP];
+    __shared__ double s_key[JBIN_CAP];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int nloc = M == N ? ncl[b] : ncl[b] / 2;
+    const int j0   = M == N ? cstart[b] : cstart[b] / 2;
+    const int g0 = gstart[b], ng = gstart[b + 1] - g0;
+    const int n  = nloc + ng;
+    if (n > JBIN_CAP) {
+        if (lane == 0) atomicOr(status, ST_JBIN_CAP);
+        return;
+    }
+    for (int e = lane; e < n; e += 64) {
+        int cj    = e < nloc ? j0 + e : ncj + glist[g0 + e - nloc];
+        s_cj[e]   = cj;
+        s_key[e]  = cj_bbox[6 * cj + 4];
+    }
+    __syncthreads();
+    const int out = jstart[b];
+    for (int e = lane; e < n; e += 64) {
+        double k = s_key[e];
+        int cj   = s_cj[e], r = 0;
+        for (int f = 0; f < n; f++)
+            r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
+        jlist[out + r] = cj;
+    }
+}
+
+// ---------------------------------------------------------------- buildNeighborCPU (neighbor.c:282-766)
+// One wavefront per i-cluster.  Candidates (stencil columns in order, clusters in column order)
+// are tested 64 at a time; ballots give every accepted j-cluster its place in the list.
+template <int M, int N>
+__global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
+    const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
+    const double* __restrict__ ci_bbox, const double* __restrict__ cj_bbox,
+    const int* __restrict__ jstart, const int* __restrict__ jlist, const double* __restrict__ sx,
+    double cutneighsq, double rbb_sq, int maxneighs, int* __restrict__ nb,
+    int* __restrict__ numneigh, int* __restrict__ nmasked, int* __restrict__ status)
+{
+    __shared__ int s_st[1024];
+    __shared__ int s_mpos[BLOCK / 64][4];
+    for (int k = threadIdx.x; k < nstencil; k += blockDim.x)
+        s_st[k] = stencil[k];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ci = blockIdx.x * (BLOCK / 64) + w;
+    if (ci >= nci) return;
+    const int ci_cj0 = (M == N) ? ci : ci >> 1;
+    const int ibin   = ci_bin[ci];
+    double ib[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        ib[k] = ci_bbox[6 * ci + k];
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int* out = nb + (size_t)ci * maxneighs;
+    int n = 0, nm = 0;
+    for (int s = 0; s < nstencil; s++) {
+        int jb = ibin + s_st[s];
+        if (jb < 0 || jb >= mbins) {
+            if (lane == 0) atomicOr(status, ST_BIN_RANGE);
+            continue;
+        }
+        const int beg = jstart[jb], end = jstart[jb + 1];
+        for (int c0 = beg; c0 < end; c0 += 64) {
+            const int m = c0 + lane;
+            bool pass = false, masked = false;
+            int cj    = -1;
+            if (m < end) {
+                cj = jlist[m];
+                if (!(half && ci_cj0 > cj)) {
+                    const double* jbb = &cj_bbox[6 * cj];
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int d = 2; d >= 0; d--) {
+                        double dl  = ib[2 * d] - jbb[2 * d + 1];
+                        double dh  = jbb[2 * d] - ib[2 * d + 1];
+                        double dm0 = fmax(fmax(dl, dh), 0.0);
+                        d2 += dm0 * dm0;
+                    }
+                    if (d2 < cutneighsq) {
+                        pass = d2 < rbb_sq;
+                        for (int cii = 0; cii < M && !pass; cii++) {
+                            const double* xi = &sx[3 * (ci * M + cii)];
+                            const double xa = xi[0], ya = xi[1], za = xi[2];
+                            for (int cjj = 0; cjj < N; cjj++) {
+                                const double* xj = &sx[3 * (cj * N + cjj)];
+                                double dx = xa - xj[0], dy = ya - xj[1], dz = za - xj[2];
+                                double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                                      : dx * dx + dy * dy + dz * dz;
+                                if (rsq < cutneighsq) {
+                                    pass = true;
+                                    break;
+                                }
+                            }
+                        }
+                    }
+                    masked = pass && imask_of(rule, ci, cj) != IMASK_ALL;
+                }
+            }
+            const unsigned long long bal  = __ballot(pass);
+            const unsigned long long mbal = __ballot(masked);
+            if (pass) {
+                int pos = n + __popcll(bal & lt);
+                if (pos < maxneighs) out[pos] = cj;
+                if (masked) {
+                    int k = nm + __popcll(mbal & lt);
+                    if (k < 4) s_mpos[w][k] = pos;
+                }
+            }
+            n += __popcll(bal);
+            nm += __popcll(mbal);
+        }
+    }
+    // single-lane epilogue: masked entries move to the front, in order of arrival
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        numneigh[ci] = n;
+        nmasked[ci]  = nm;
+        if (n > maxneighs || nm > 4) {
+            atomicOr(status, ST_NB_OVERFLOW);
+            atomicMax(&status[1], n);
+        } else {
+            for (int k = 0; k < nm; k++) {
+                int p   = s_mpos[w][k];
+                int t   = out[k];
+                out[k]  = out[p];
+                out[p]  = t;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 force, reference arithmetic
+template <int M, int N, bool HALF, bool EV>
+__global__ void __launch_bounds__(BLOCK) k_cp_force_f64(int nci, int ncj, int ntypes, int uniform,
+    Tab64 tab, const double* __restrict__ sx, const int* __restrict__ st,
+    const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
+    double* __restrict__ f, double* __restrict__ ev)
+{
+    __shared__ double s_red[2][BLOCK / 64];
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ci = si / M;
+    double e = 0.0, w = 0.0;
+    if (ci < nci) {
+        const int ci_cj0 = (M == N) ? ci : ci >> 1;
+        const double xt = sx[3 * si], yt = sx[3 * si + 1], zt = sx[3 * si + 2];
+        const int ti    = st[si];
+        const int* lst  = nb + (size_t)ci * maxneighs;
+        const int nn    = numneigh[ci];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int k = 0; k < nn; k++) {
+            const int cj = lst[k];
+            double fix = 0, fiy = 0, fiz = 0;
+            for (int cjj = 0; cjj < N; cjj++) {
+                const int sj = cj * N + cjj;
+                bool cond    = HALF ? (ci_cj0 != cj || si < sj) : (ci_cj0 != cj || si != sj);
+                if (!cond) continue;
+                double delx = xt - sx[3 * sj];
+                double dely = yt - sx[3 * sj + 1];
+                double delz = zt - sx[3 * sj + 2];
+                double rsq  = delx * delx + dely * dely + delz * delz;
+                int tij     = uniform ? 0 : ti * ntypes + st[sj];
+                if (rsq < tab.cutf[tij]) {
+                    double sr2   = 1.0 / rsq;
+                    double sr6   = sr2 * sr2 * sr2 * tab.sig6[tij];
+                    double force = 48.0 * sr6 * (sr6 - 0.5) * sr2 * tab.eps[tij];
+                    if (HALF && cj < ncj) {
+                        atomicAdd(&f[3 * sj], -(delx * force));
+                        atomicAdd(&f[3 * sj + 1], -(dely * force));
+                        atomicAdd(&f[3 * sj + 2], -(delz * force));
+                    }
+                    fix += delx * force;
+                    fiy += dely * force;
+                    fiz += delz * force;
+                    if (EV) {
+                        double wgt = (HALF && cj < ncj) ? 1.0 : 0.5;
+                        e += wgt * 4.0 * tab.eps[tij] * sr6 * (sr6 - 1.0);
+                        w += wgt * rsq * force;
+                    }
+                }
+            }
+            fx += fix;
+            fy += fiy;
+            fz += fiz;
+        }
+        if (HALF) {
+            atomicAdd(&f[3 * si], fx);
+            atomicAdd(&f[3 * si + 1], fy);
+            atomicAdd(&f[3 * si + 2], fz);
+        } else {
+            f[3 * si]     = fx;
+            f[3 * si + 1] = fy;
+            f[3 * si + 2] = fz;
+        }
+    }
+    if (EV) {
+        for (int o = 32; o > 0; o >>= 1) {
+            e += __shfl_xor(e, o, 64);
+            w += __shfl_xor(w, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&ev[0], e);
+            atomicAdd(&ev[1], w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32-mixed force kernel
+__device__ inline void kahan_add(float& s, float& c, float t)
+{
+    float y  = t - c;
+    float tt = s + y;
+    c        = (tt - s) - y;
+    s        = tt;
+}
+
+// (s, c) += (s2, c2) with the rounding error of s + s2 kept (TwoSum).
+__device__ inline void two_sum_merge(float& s, float& c, float s2, float c2)
+{
+    float t  = s + s2;
+    float bp = t - s;
+    float er = (s - (t - bp)) + (s2 - bp);
+    s        = t;
+    c        = c + c2 + er;
+}
+
+template <int N, bool HALF, bool EV, bool UNIFORM, bool COMP>
+__global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int ntypes, Tab32 tab,
+    const double* __restrict__ sx, const int* __restrict__ st, const int* __restrict__ nb,
+    const int* __restrict__ numneigh, const int* __restrict__ ci_natoms, int maxneighs,
+    float* __restrict__ f, double* __restrict__ ev)
+{
+    constexpr int CPI = 16 / N;  // j-clusters per iteration
+    __shared__ float s_tab[4][MAXT2];
+    if (!UNIFORM) {
+        for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
+            s_tab[0][k] = tab.eps48[k];
+            s_tab[1][k] = tab.eps4[k];
+            s_tab[2][k] = tab.sig6[k];
+            s_tab[3][k] = tab.cut2[k];
+        }
+        __syncthreads();
+    }
+    // XCD-aware block order: the 8 XCDs receive blocks round-robin; give each XCD a contiguous
+    // range of i-clusters so that neighbouring i-clusters share that XCD's L2.  The grid is a
+    // multiple of 8 blocks, which makes the remapping a permutation.
+    const int per = gridDim.x >> 3;
+    const int lb  = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int ci  = lb * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (ci >= nci) return;
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 3, s = lane >> 2;
+    const int g = s / N, jj = s % N;
+    const int si     = ci * 4 + a;
+    const bool ilive = a < ci_natoms[ci];
+    const double xi = sx[3 * si], yi = sx[3 * si + 1], zi = sx[3 * si + 2];
+    const int ci_cj0 = (N == 4) ? ci : ci >> 1;
+    const int ti     = UNIFORM ? 0 : st[si];
+    const int nn     = numneigh[ci];
+    const int* lst   = nb + (size_t)ci * maxneighs;
+    const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0], cut2u = tab.cut2[0];
+    float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    float e = 0.f, w = 0.f;
+    for (int k0 = 0; k0 < nn; k0 += CPI) {
+        const int k      = k0 + g;
+        const bool valid = k < nn;
+        const int cj     = valid ? lst[k] : ci_cj0;
+        const int sj     = cj * N + jj;
+        const float dx   = (float)(xi - sx[3 * sj]);
+        const float dy   = (float)(yi - sx[3 * sj + 1]);
+        const float dz   = (float)(zi - sx[3 * sj + 2]);
+        const float rsq  = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u, cut2 = cut2u;
+        if (!UNIFORM) {
+            const int tij = ti * ntypes + st[sj];
+            eps48         = s_tab[0][tij];
+            eps4          = s_tab[1][tij];
+            sig6          = s_tab[2][tij];
+            cut2          = s_tab[3][tij];
+        }
+        const bool excl = (cj == ci_cj0) && (HALF ? !(si < sj) : (si == sj));
+        const bool in   = valid && ilive && !excl && rsq < cut2;
+        float r2i       = __builtin_amdgcn_rcpf(rsq);
+        r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
+        const float sr6 = r2i * r2i * r2i * sig6;
+        float fpair     = eps48 * sr6 * (sr6 - 0.5f) * r2i;
+        fpair           = in ? fpair : 0.0f;
+        if (COMP) {
+            kahan_add(fx, cx, dx * fpair);
+            kahan_add(fy, cy, dy * fpair);
+            kahan_add(fz, cz, dz * fpair);
+        } else {
+            fx = fmaf(dx, fpair, fx);
+            fy = fmaf(dy, fpair, fy);
+            fz = fmaf(dz, fpair, fz);
+        }
+        if (EV) {
+            const float wgt = (HALF && cj < ncj) ? 1.0f : 0.5f;
+            e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
+            w += wgt * rsq * fpair;
+        }
+        if (HALF) {
+            // j-force: reduce over the four i-lanes that share this j-atom, one atomic per atom
+            float jx = -dx * fpair, jy = -dy * fpair, jz = -dz * fpair;
+            jx += __shfl_xor(jx, 1, 64);
+            jy += __shfl_xor(jy, 1, 64);
+            jz += __shfl_xor(jz, 1, 64);
+            jx += __shfl_xor(jx, 2, 64);
+            jy += __shfl_xor(jy, 2, 64);
+            jz += __shfl_xor(jz, 2, 64);
+            if (a == 0 && valid && cj < ncj && (jx != 0.f || jy != 0.f || jz != 0.f)) {
+                atomicAdd(&f[3 * sj], jx);
+                atomicAdd(&f[3 * sj + 1], jy);
+                atomicAdd(&f[3 * sj + 2], jz);
+            }
+        }
+    }
+    // reduce the 16 j-slot lanes of each i-atom (lane bits 2..5)
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+        float ox = __shfl_xor(fx, o, 64), oy = __shfl_xor(fy, o, 64), oz = __shfl_xor(fz, o, 64);
+        if (COMP) {
+            float ocx = __shfl_xor(cx, o, 64), ocy = __shfl_xor(cy, o, 64),
+                  ocz = __shfl_xor(cz, o, 64);
+            two_sum_merge(fx, cx, ox, ocx);
+            two_sum_merge(fy, cy, oy, ocy);
+            two_sum_merge(fz, cz, oz, ocz);
+        } else {
+            fx += ox;
+            fy += oy;
+            fz += oz;
+        }
+    }
+    if (s == 0 && ilive) {
+        const float rx = COMP ? fx + cx : fx, ry = COMP ? fy + cy : fy, rz = COMP ? fz + cz : fz;
+        if (HALF) {
+            atomicAdd(&f[3 * si], rx);
+            atomicAdd(&f[3 * si + 1], ry);
+            atomicAdd(&f[3 * si + 2], rz);
+        } else {
+            f[3 * si]     = rx;
+            f[3 * si + 1] = ry;
+            f[3 * si + 2] = rz;
+        }
+    }
+    if (EV) {
+        double ed = e, wd = w;
+        for (int o = 32; o > 0; o >>= 1) {
+            ed += __shfl_xor(ed, o, 64);
+            wd += __shfl_xor(wd, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&ev[0], ed);
+            atomicAdd(&ev[1], wd);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- integrate (integrate.c:29-86)
+template <class FT>
+__global__ void k_cp_initial(int n3, int M, const int* __restrict__ ci_natoms, double* __restrict__ x,
+    double* __restrict__ v, const FT* __restrict__ f, double dt, double dtforce)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n3) return;
+    int s = k / 3;
+    if (s % M >= ci_natoms[s / M]) return;
+    double vv = v[k] + dtforce * (double)f[k];
+    v[k]      = vv;
+    x[k]      = x[k] + dt * vv;
+}
+
+template <class FT>
+__global__ void k_cp_final(int n3, int M, const int* __restrict__ ci_natoms, double* __restrict__ v,
+    const FT* __restrict__ f, double dtforce)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n3) return;
+    int s = k / 3;
+    if (s % M >= ci_natoms[s / M]) return;
+    v[k] = v[k] + dtforce * (double)f[k];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cp_ksum(int nslot, int M, const int* __restrict__ ci_natoms,
+    const double* __restrict__ v, double mass, double* __restrict__ part)
+{
+    __shared__ double sh[BLOCK / 64];
+    double t = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslot; s += gridDim.x * blockDim.x)
+        if (s % M < ci_natoms[s / M])
+            t += (v[3 * s] * v[3 * s] + v[3 * s + 1] * v[3 * s + 1] + v[3 * s + 2] * v[3 * s + 2]) *
+                 mass;
+    for (int o = 32; o > 0; o >>= 1)
+        t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int k = 0; k < BLOCK / 64; k++)
+            r += sh[k];
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ void k_cp_sum_partials(int n, double* __restrict__ part)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double r = 0;
+        for (int k = 0; k < n; k++)
+            r += part[k];
+        part[0] = r;
+    }
+}
+
+// ---------------------------------------------------------------- host helpers
+bool ok() { return !failed() && C.ready; }
+
+void read_status()
+{
+    MDG_HIP(hipMemcpyAsync(C.h_status, C.d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+}
+
+int read_int(const int* d)
+{
+    MDG_HIP(hipMemcpyAsync(C.h_status + 8, d, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    return C.h_status[8];
+}
+
+size_t nslots_all() { return ((size_t)C.ncj + C.nghost + 1) * C.N; }
+
+bool ensure_slots(size_t nslot, size_t keep_slots)
+{
+    grow_n(&C.d_sx, &C.cap_sx, 3 * nslot, 3 * keep_slots);
+    grow_n(&C.d_st, &C.cap_st, nslot, keep_slots);
+    return !failed();
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int setClusterShapeGFX950(int M, int N)
+{
+    if (!((M == 4 && N == 4) || (M == 4 && N == 8) || (M == 1 && N == 2))) {
+        set_error(E_ARG, "cluster shape %dx%d not supported (4x4, 4x8, 1x2)", M, N);
+        return -1;
+    }
+    C.M        = M;
+    C.N        = N;
+    C.rule     = (M == 1) ? 2 : N;  // get_imask_simd_j2/j4/j8 of the matching reference build
+    C.fma_dist = (M == 4) ? 1 : 0;  // the 4xN SIMD list test uses FMA; the plain-C one does not
+    return 0;
+}
+
+void setClusterPrecisionGFX950(int precision, int compensated)
+{
+    C.precision = precision;
+    C.comp      = compensated;
+}
+
+void initDeviceClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
+{
+    if (failed() || !stream()) return;
+    if (C.precision == 0 && C.M != 4) {
+        set_error(E_ARG, "the FP32 kernel needs i-clusters of 4 atoms");
+        return;
+    }
+    if (!C.d_status) {
+        MDG_HIP(hipMalloc(&C.d_status, 16 * sizeof(int)));
+        MDG_HIP(hipHostMalloc(&C.h_status, 16 * sizeof(int), hipHostMallocDefault));
+        MDG_HIP(hipMalloc(&C.d_ev, 4 * sizeof(double)));
+        MDG_HIP(hipHostMalloc(&C.h_red, 4 * sizeof(double), hipHostMallocDefault));
+        if (failed()) return;
+    }
+    fill_i32(C.d_status, 0, 16);
+    C.natoms   = atom->Nlocal;
+    C.ntypes   = atom->ntypes;
+    C.cutforce = param->cutforce;
+    C.cutneigh = param->cutneigh;
+    C.dt       = param->dt;
+    C.dtforce  = param->dtforce;
+    C.mass     = param->mass;
+    if (C.ntypes * C.ntypes > MAXT2) {
+        set_error(E_ARG, "ntypes %d > 8 not supported", C.ntypes);
+        return;
+    }
+    C.uniform = true;
+    for (int k = 0; k < C.ntypes * C.ntypes; k++) {
+        C.t64.eps[k]    = atom->epsilon[k];
+        C.t64.sig6[k]   = atom->sigma6[k];
+        C.t64.cutf[k]   = atom->cutforcesq[k];
+        C.t32.eps48[k]  = (float)(48.0 * atom->epsilon[k]);
+        C.t32.eps4[k]   = (float)(4.0 * atom->epsilon[k]);
+        C.t32.sig6[k]   = (float)atom->sigma6[k];
+        C.t32.cut2[k]   = (float)atom->cutforcesq[k];
+        if (atom->epsilon[k] != atom->epsilon[0] || atom->sigma6[k] != atom->sigma6[0] ||
+            atom->cutforcesq[k] != atom->cutforcesq[0])
+            C.uniform = false;
+    }
+    C.maxneighs = neighbor->maxneighs > 0 ? neighbor->maxneighs : 200;
+    C.ready     = true;
+    copyAtomsToClusterGFX950Device(atom);
+}
+
+void freeDeviceClusterGFX950(void)
+{
+    if (C.d_status) (void)hipStreamSynchronize(stream());
+    void* ptrs[] = { C.d_stencil, C.d_ax, C.d_av, C.d_at, C.d_abin_of, C.d_abin_cnt,
+        C.d_abin_start, C.d_abin_atoms, C.d_ncl, C.d_cstart, C.d_ci_natoms, C.d_ci_bin,
+        C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
+        C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
+        C.d_jlist, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb, C.d_numneigh, C.d_nmasked,
+        C.d_red, C.d_ev, C.d_status };
+    for (void* p : ptrs)
+        dfree(p);
+    if (C.h_status) (void)hipHostFree(C.h_status);
+    if (C.h_red) (void)hipHostFree(C.h_red);
+    int M = C.M, N = C.N, rule = C.rule, fm = C.fma_dist, prec = C.precision, comp = C.comp;
+    C = CPState();
+    C.M = M;
+    C.N = N;
+    C.rule = rule;
+    C.fma_dist = fm;
+    C.precision = prec;
+    C.comp = comp;
+}
+
+void copyAtomsToClusterGFX950Device(Atom* atom)
+{
+    if (!ok()) return;
+    size_t n = (size_t)atom->Nlocal;
+    C.natoms = atom->Nlocal;
+    grow_n(&C.d_ax, &C.cap_ax, 3 * n + 3, 0);
+    grow_n(&C.d_av, &C.cap_av, 3 * n + 3, 0);
+    grow_n(&C.d_at, &C.cap_at, n + 1, 0);
+    if (failed()) return;
+    double* v = (double*)malloc(sizeof(double) * 3 * n + 8);
+    for (size_t i = 0; i < n; i++) {
+        v[3 * i]     = atom->vx[i];
+        v[3 * i + 1] = atom->vy[i];
+        v[3 * i + 2] = atom->vz[i];
+    }
+    MDG_HIP(hipMemcpyAsync(C.d_ax, atom->x, 3 * n * sizeof(double), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipMemcpyAsync(C.d_av, v, 3 * n * sizeof(double), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipMemcpyAsync(C.d_at, atom->type, n * sizeof(int), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    free(v);
+}
+
+void copyAtomsFromClusterGFX950Device(Atom* atom)
+{
+    if (!ok()) return;
+    size_t n = (size_t)C.natoms;
+    double* v = (double*)malloc(sizeof(double) * 3 * n + 8);
+    MDG_HIP(hipMemcpyAsync(atom->x, C.d_ax, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipMemcpyAsync(v, C.d_av, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipMemcpyAsync(atom->type, C.d_at, n * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    for (size_t i = 0; i < n; i++) {
+        atom->vx[i] = v[3 * i];
+        atom->vy[i] = v[3 * i + 1];
+        atom->vz[i] = v[3 * i + 2];
+    }
+    free(v);
+}
+
+// setupNeighbor (src/clusterpair/neighbor.c:108-215), restated on the host.
+void setupNeighborClusterGFX950(Parameter* param, Atom* atom)
+{
+    if (!ok()) return;
+    const double SMALL = 1.0e-6, FACTOR = 0.999;
+    double xprd = param->xprd, yprd = param->yprd, zprd = param->zprd;
+    double xlo = 0.0, xhi = xprd, ylo = 0.0, yhi = yprd, zlo = 0.0, zhi = zprd;
+    double atom_density  = ((double)(atom->Natoms)) / ((xhi - xlo) * (yhi - ylo) * (zhi - zlo));
+    double atoms_in_cell = C.M > C.N ? C.M : C.N;
+    double tsx = cbrt(atoms_in_cell / atom_density), tsy = cbrt(atoms_in_cell / atom_density);
+    CGrid& g   = C.g;
+    g.xprd = xprd;
+    g.yprd = yprd;
+    g.zprd = zprd;
+    int t  = (int)ceil((xhi - xlo) / tsx);
+    g.nbinx = t > 1 ? t : 1;
+    t       = (int)ceil((yhi - ylo) / tsy);
+    g.nbiny = t > 1 ? t : 1;
+    C.binsizex   = (xhi - xlo) / g.nbinx;
+    C.binsizey   = (yhi - ylo) / g.nbiny;
+    g.bininvx    = 1.0 / C.binsizex;
+    g.bininvy    = 1.0 / C.binsizey;
+    C.cutneigh   = param->cutneigh;
+    C.cutneighsq = C.cutneigh * C.cutneigh;
+    double coord = xlo - C.cutneigh - SMALL * xprd;
+    g.mbinxlo    = (int)(coord * g.bininvx);
+    if (coord < 0.0) g.mbinxlo = g.mbinxlo - 1;
+    coord       = xhi + C.cutneigh + SMALL * xprd;
+    int mbinxhi = (int)(coord * g.bininvx);
+    coord       = ylo - C.cutneigh - SMALL * yprd;
+    g.mbinylo   = (int)(coord * g.bininvy);
+    if (coord < 0.0) g.mbinylo = g.mbinylo - 1;
+    coord       = yhi + C.cutneigh + SMALL * yprd;
+    int mbinyhi = (int)(coord * g.bininvy);
+    g.mbinxlo   = g.mbinxlo - 1;
+    mbinxhi     = mbinxhi + 1;
+    g.mbinx     = mbinxhi - g.mbinxlo + 1;
+    g.mbinylo   = g.mbinylo - 1;
+    mbinyhi     = mbinyhi + 1;
+    g.mbiny     = mbinyhi - g.mbinylo + 1;
+    g.mbins     = g.mbinx * g.mbiny;
+    int nextx   = (int)(C.cutneigh * g.bininvx);
+    int nexty   = (int)(C.cutneigh * g.bininvy);
+    if (nextx * C.binsizex < FACTOR * C.cutneigh) nextx++;
+    if (nexty * C.binsizey < FACTOR * C.cutneigh) nexty++;
+    int st[1024], ns = 0;
+    for (int j = -nexty; j <= nexty; j++)
+        for (int i = -nextx; i <= nextx; i++) {
+            double dx = i > 0 ? (i - 1) * C.binsizex : (i == 0 ? 0.0 : (i + 1) * C.binsizex);
+            double dy = j > 0 ? (j - 1) * C.binsizey : (j == 0 ? 0.0 : (j + 1) * C.binsizey);
+            if (dx * dx + dy * dy < C.cutneighsq) {
+                if (ns >= 1024) {
+                    set_error(E_ARG, "cluster stencil larger than 1024 columns");
+                    return;
+                }
+                st[ns++] = j * g.mbinx + i;
+            }
+        }
+    C.nstencil = ns;
+    double bbx = 0.5 * (C.binsizex + C.binsizex), bby = 0.5 * (C.binsizey + C.binsizey);
+    double r   = C.cutneigh - 0.5 * sqrt(bbx * bbx + bby * bby);
+    r          = r > 0.0 ? r : 0.0;
+    C.rbb_sq   = r * r;
+    grow_n(&C.d_stencil, &C.cap_stencil, (size_t)ns, 0);
+    if (failed()) return;
+    MDG_HIP(hipMemcpyAsync(C.d_stencil, st, ns * sizeof(int), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+}
+
+void updateSingleAtomsGFX950(Parameter* param, Atom* atom)
+{
+    (void)param;
+    if (!ok() || C.nci == 0) return;
+    TimedRegion tr(F_CLUSTER);
+    grow_n(&C.d_aoff, &C.cap_aoff, (size_t)C.nci + 1, 0);
+    if (failed()) return;
+    hipStream_t s = stream();
+    exclusive_scan(C.d_ci_natoms, C.d_aoff, C.nci, C.d_status + 4);
+    int ns = C.nci * C.M;
+    k_cp_slots_to_atoms<<<blocks_for(ns, BLOCK), BLOCK, 0, s>>>(C.nci, C.M, C.d_ci_natoms, C.d_aoff,
+        C.d_sx, C.d_sv, C.d_st, C.d_ax, C.d_av, C.d_at);
+    MDG_LAUNCH_CHECK("slots_to_atoms");
+    (void)atom;
+}
+
+void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy)
+{
+    (void)dummy;
+    (void)atom;
+    if (!ok()) return;
+    TimedRegion tr(F_PBC);
+    int n3 = 3 * C.natoms;
+    k_cp_wrap<<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.d_ax, param->xprd, param->yprd,
+        param->zprd);
+    MDG_LAUNCH_CHECK("cp_wrap");
+}
+
+void buildClustersGFX950(Atom* atom)
+{
+    if (!ok()) return;
+    if (C.g.mbins == 0) {
+        set_error(E_ARG, "buildClustersGFX950 before setupNeighborClusterGFX950");
+        return;
+    }
+    TimedRegion tr(F_CLUSTER);
+    hipStream_t s = stream();
+    const int na = C.natoms, mb = C.g.mbins;
+    grow_n(&C.d_abin_of, &C.cap_abin_of, (size_t)na + 1, 0);
+    grow_n(&C.d_abin_cnt, &C.cap_abin_cnt, (size_t)mb + 1, 0);
+    grow_n(&C.d_abin_start, &C.cap_abin_start, (size_t)mb + 1, 0);
+    grow_n(&C.d_abin_atoms, &C.cap_abin_atoms, (size_t)na + 1, 0);
+    grow_n(&C.d_ncl, &C.cap_ncl, (size_t)mb + 1, 0);
+    grow_n(&C.d_cstart, &C.cap_cstart, (size_t)mb + 1, 0);
+    if (failed()) return;
+    fill_i32(C.d_status, 0, 8);
+    fill_i32(C.d_abin_cnt, 0, (size_t)mb + 1);
+    k_cp_bin_count<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_ax, C.g, C.d_abin_of,
+        C.d_abin_cnt, C.d_status);
+    exclusive_scan(C.d_abin_cnt, C.d_abin_start, mb, C.d_abin_start + mb);
+    fill_i32(C.d_abin_cnt, 0, (size_t)mb);
+    k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
+        C.d_abin_cnt, C.d_abin_atoms);
+    k_cp_bin_sort<<<mb, 64, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
+        C.d_status);
+    exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb);
+    MDG_LAUNCH_CHECK("cp_binning");
+    read_status();
+    if (failed()) return;
+    if (C.h_status[0] & (ST_BIN_RANGE | ST_BIN_CAP)) {
+        set_error(E_BIN, "cluster binning failed (flags %d): an atom outside the grid or a "
+                         "column above %d atoms", C.h_status[0], BIN_CAP);
+        return;
+    }
+    int nci = read_int(C.d_cstart + mb);
+    if (failed()) return;
+    C.nci = nci;
+    C.ncj = nci * C.M / C.N;
+    grow_n(&C.d_ci_natoms, &C.cap_ci_natoms, (size_t)nci + 1, 0);
+    grow_n(&C.d_ci_bin, &C.cap_ci_bin, (size_t)nci + 1, 0);
+    grow_n(&C.d_ci_bbox, &C.cap_ci_bbox, 6 * (size_t)nci + 6, 0);
+    grow_n(&C.d_sv, &C.cap_sv, 3 * (size_t)nci * C.M + 3, 0);
+    ensure_slots((size_t)nci * C.M + C.N, 0);
+    if (failed()) return;
+    k_cp_fill_ci_bin<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.d_cstart, C.d_ncl, C.d_ci_bin);
+    k_cp_fill_clusters<<<blocks_for(nci, BLOCK), BLOCK, 0, s>>>(nci, C.M, C.d_ci_bin, C.d_cstart,
+        C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx, C.d_sv, C.d_st,
+        C.d_ci_natoms, C.d_ci_bbox);
+    MDG_LAUNCH_CHECK("cp_fill_clusters");
+    atom->Nclusters_local = nci;
+}
+
+void defineJClustersGFX950(Parameter* param, Atom* atom)
+{
+    (void)param;
+    if (!ok()) return;
+    TimedRegion tr(F_CLUSTER);
+    // ghosts follow: reserve their descriptors later, now the local ones
+    grow_n(&C.d_cj_natoms, &C.cap_cj_natoms, (size_t)C.ncj + 1, 0);
+    grow_n(&C.d_cj_bbox, &C.cap_cj_bbox, 6 * (size_t)C.ncj + 6, 0);
+    if (failed()) return;
+    k_cp_define_j<<<blocks_for(C.ncj, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.M, C.N, C.d_ci_natoms,
+        C.d_ci_bbox, C.d_cj_natoms, C.d_cj_bbox);
+    MDG_LAUNCH_CHECK("cp_define_j");
+    atom->ncj = C.ncj;
+}
+
+void setupPbcClusterGFX950(Atom* atom, Parameter* param)
+{
+    if (!ok()) return;
+    TimedRegion tr(F_PBC);
+    hipStream_t s = stream();
+    const int ncj = C.ncj;
+    grow_n(&C.d_gcnt, &C.cap_gcnt, (size_t)ncj + 1, 0);
+    grow_n(&C.d_goff, &C.cap_goff, (size_t)ncj + 1, 0);
+    if (failed()) return;
+    k_cp_ghost_count<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.d_cj_natoms, C.d_cj_bbox,
+        param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_gcnt);
+    exclusive_scan(C.d_gcnt, C.d_goff, ncj, C.d_status + 2);
+    fill_i32(C.d_status + 3, 0, 1);
+    read_status();
+    if (failed()) return;
+    const int ng = C.h_status[2];
+    C.nghost     = ng;
+    C.dummy_cj   = ncj + ng;
+    size_t nj    = (size_t)ncj + ng + 1;
+    grow_n(&C.d_cj_natoms, &C.cap_cj_natoms, nj, (size_t)ncj);
+    grow_n(&C.d_cj_bbox, &C.cap_cj_bbox, 6 * nj, 6 * (size_t)ncj);
+    grow_n(&C.d_border, &C.cap_border, (size_t)ng + 1, 0);
+    grow_n(&C.d_pbc, &C.cap_pbc, 3 * (size_t)ng + 3, 0);
+    ensure_slots(nj * C.N, (size_t)ncj * C.N);
+    if (failed()) return;
+    k_cp_ghost_fill<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.N, C.d_cj_natoms, C.d_cj_bbox,
+        param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_goff, C.d_border, C.d_pbc,
+        C.d_cj_natoms, C.d_st, C.d_status);
+    k_cp_ghost_first<<<blocks_for(ng + 1, BLOCK), BLOCK, 0, s>>>(ncj, ng, C.N, C.d_border, C.d_pbc,
+        C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx, C.d_st, C.d_cj_bbox);
+    MDG_LAUNCH_CHECK("cp_setup_pbc");
+    read_status();
+    C.nghost_atoms        = C.h_status[3];
+    atom->Nclusters_ghost = ng;
+    atom->dummy_cj        = C.dummy_cj;
+    atom->Nghost          = C.nghost_atoms;
+}
+
+void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate)
+{
+    (void)atom;
+    if (!ok() || C.nghost == 0) return;
+    TimedRegion tr(F_PBC);
+    if (firstUpdate) {
+        k_cp_ghost_first<<<blocks_for(C.nghost + 1, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost,
+            C.N, C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx,
+            C.d_st, C.d_cj_bbox);
+    } else {
+        long long n = 3LL * C.nghost * C.N;
+        k_cp_ghost_update<<<blocks_for(n, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost, C.N,
+            C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx);
+    }
+    MDG_LAUNCH_CHECK("cp_update_pbc");
+}
+
+void binJClustersGFX950(Parameter* param, Atom* atom)
+{
+    (void)param;
+    (void)atom;
+    if (!ok()) return;
+    TimedRegion tr(F_CLUSTER);
+    hipStream_t s = stream();
+    const int mb = C.g.mbins, ng = C.nghost;
+    grow_n(&C.d_gbin, &C.cap_gbin, (size_t)ng + 1, 0);
+    grow_n(&C.d_gcur, &C.cap_gcur, (size_t)mb + 1, 0);
+    grow_n(&C.d_gbin_start, &C.cap_gbin_start, (size_t)mb + 1, 0);
+    grow_n(&C.d_gbin_list, &C.cap_gbin_list, (size_t)ng + 1, 0);
+    grow_n(&C.d_jcnt, &C.cap_jcnt, (size_t)mb + 1, 0);
+    grow_n(&C.d_jstart, &C.cap_jstart, (size_t)mb + 1, 0);
+    grow_n(&C.d_jlist, &C.cap_jlist, (size_t)C.ncj + ng + 1, 0);
+    if (failed()) return;
+    fill_i32(C.d_status, 0, 1);
+    fill_i32(C.d_gcur, 0, (size_t)mb + 1);
+    k_cp_ghost_bin<<<blocks_for(ng, BLOCK), BLOCK, 0, s>>>(C.ncj, ng, C.N, C.g, C.d_cj_natoms,
+        C.d_sx, C.d_pbc, C.d_gbin, C.d_gcur, C.d_status);
+    exclusive_scan(C.d_gcur, C.d_gbin_start, mb, C.d_gbin_start + mb);
+    k_cp_jbin_count<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.M, C.N, C.d_ncl, C.d_gcur, C.d_jcnt);
+    exclusive_scan(C.d_jcnt, C.d_jstart, mb, C.d_jstart + mb);
+    fill_i32(C.d_gcur, 0, (size_t)mb + 1);
+    k_cp_ghost_scatter<<<blocks_for(ng, BLOCK), BLOCK, 0, s>>>(ng, C.d_gbin, C.d_gbin_start,
+        C.d_gcur, C.d_gbin_list);
+    k_cp_jbin_merge<<<mb, 64, 0, s>>>(C.M, C.N, C.ncj, C.d_ncl, C.d_cstart, C.d_gbin_start,
+        C.d_gbin_list, C.d_cj_bbox, C.d_jstart, C.d_jlist, C.d_status);
+    MDG_LAUNCH_CHECK("cp_bin_j");
+}
+
+void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
+{
+    if (!ok()) return;
+    TimedRegion tr(F_NEIGHBOR);
+    hipStream_t s = stream();
+    const int nci = C.nci;
+    if (neighbor->maxneighs > C.maxneighs) C.maxneighs = neighbor->maxneighs;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        grow_n(&C.d_nb, &C.cap_nb, (size_t)nci * C.maxneighs + 1, 0);
+        grow_n(&C.d_numneigh, &C.cap_numneigh, (size_t)nci + 1, 0);
+        grow_n(&C.d_nmasked, &C.cap_nmasked, (size_t)nci + 1, 0);
+        if (failed()) return;
+        fill_i32(C.d_status, 0, 2);
+        unsigned grid = blocks_for(nci, BLOCK / 64);
+#define MDG_NB(MM, NN)                                                                           \
+    k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
+        C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_cj_bbox, C.d_jstart,      \
+        C.d_jlist, C.d_sx, C.cutneighsq, C.rbb_sq, C.maxneighs, C.d_nb, C.d_numneigh,             \
+        C.d_nmasked, C.d_status)
+        if (C.M == 4 && C.N == 4) MDG_NB(4, 4);
+        else if (C.M == 4 && C.N == 8) MDG_NB(4, 8);
+        else MDG_NB(1, 2);
+#undef MDG_NB
+        MDG_LAUNCH_CHECK("cp_build_nb");
+        read_status();
+        if (failed()) return;
+        if (C.h_status[0] & (ST_BIN_RANGE | ST_JBIN_CAP)) {
+            set_error(E_BIN, "cluster-pair list build failed (flags %d)", C.h_status[0]);
+            return;
+        }
+        if (!(C.h_status[0] & ST_NB_OVERFLOW)) {
+            neighbor->maxneighs = C.maxneighs;
+            (void)atom;
+            return;
+        }
+        // neighbor.c:705-720: grow to 1.2x the largest count and rebuild
+        C.maxneighs = (int)(C.h_status[1] * 1.2) + 1;
+    }
+    set_error(E_CAPACITY, "cluster-pair list did not converge");
+}
+
+double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
+{
+    (void)atom;
+    (void)stats;
+    if (!ok()) return 0.0;
+    const double t0 = wtime();
+    hipStream_t s   = stream();
+    const int nci = C.nci, half = neighbor->half_neigh;
+    const size_t nloc = (size_t)nci * C.M;
+    if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
+    TimedRegion tr(F_FORCE_CP);
+    if (C.precision == 1) {
+        grow_n(&C.d_sf64, &C.cap_sf64, 3 * nslots_all(), 0);
+        if (failed()) return 0.0;
+        if (half) fill_f64(C.d_sf64, 0.0, 3 * nslots_all());
+        unsigned grid = blocks_for((long long)nloc, BLOCK);
+        const int uni = C.uniform ? 1 : 0;
+#define MDG_F64(MM, NN, H, E)                                                                    \
+    k_cp_force_f64<MM, NN, H, E><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.ntypes, uni, C.t64, C.d_sx, \
+        C.d_st, C.d_nb, C.d_numneigh, C.maxneighs, C.d_sf64, C.d_ev)
+#define MDG_F64_SHAPE(MM, NN)                                                                    \
+    do {                                                                                         \
+        if (half) {                                                                              \
+            if (C.ev_on) MDG_F64(MM, NN, true, true);                                            \
+            else MDG_F64(MM, NN, true, false);                                                   \
+        } else {                                                                                 \
+            if (C.ev_on) MDG_F64(MM, NN, false, true);                                           \
+            else MDG_F64(MM, NN, false, false);                                                  \
+        }                                                                                        \
+    } while (0)
+        if (C.M == 4 && C.N == 4) MDG_F64_SHAPE(4, 4);
+        else if (C.M == 4 && C.N == 8) MDG_F64_SHAPE(4, 8);
+        else MDG_F64_SHAPE(1, 2);
+#undef MDG_F64_SHAPE
+#undef MDG_F64
+        C.forces_are_f32 = false;
+    } else {
+        grow_n(&C.d_sf32, &C.cap_sf32, 3 * nslots_all(), 0);
+        if (failed()) return 0.0;
+        if (half) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
+        unsigned grid = (blocks_for(nci, BLOCK / 64) + 7) & ~7u;  // multiple of 8, see kernel
+#define MDG_F32(NN, H, E, U, CO)                                                                 \
+    k_cp_force_f32<NN, H, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.ntypes, C.t32, C.d_sx,   \
+        C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms, C.maxneighs, C.d_sf32, C.d_ev)
+#define MDG_F32_U(NN, H, E)                                                                      \
+    do {                                                                                         \
+        if (C.uniform) {                                                                         \
+            if (C.comp) MDG_F32(NN, H, E, true, true);                                           \
+            else MDG_F32(NN, H, E, true, false);                                                 \
+        } else {                                                                                 \
+            if (C.comp) MDG_F32(NN, H, E, false, true);                                          \
+            else MDG_F32(NN, H, E, false, false);                                                \
+        }                                                                                        \
+    } while (0)
+#define MDG_F32_N(NN)                                                                            \
+    do {                                                                                         \
+        if (half) {                                                                              \
+            if (C.ev_on) MDG_F32_U(NN, true, true);                                              \
+            else MDG_F32_U(NN, true, false);                                                     \
+        } else {                                                                                 \
+            if (C.ev_on) MDG_F32_U(NN, false, true);                                             \
+            else MDG_F32_U(NN, false, false);                                                    \
+        }                                                                                        \
+    } while (0)
+        if (C.N == 4) MDG_F32_N(4);
+        else MDG_F32_N(8);
+#undef MDG_F32_N
+#undef MDG_F32_U
+#undef MDG_F32
+        C.forces_are_f32 = true;
+    }
+    MDG_LAUNCH_CHECK("cp_force");
+    (void)param;
+    return wtime() - t0;
+}
+
+void initialIntegrateClusterGFX950(Parameter* param, Atom* atom)
+{
+    (void)atom;
+    if (!ok()) return;
+    TimedRegion tr(F_INTEGRATE);
+    int n3 = 3 * C.nci * C.M;
+    if (!(C.forces_are_f32 ? (void*)C.d_sf32 : (void*)C.d_sf64)) {
+        set_error(E_ARG, "initialIntegrateClusterGFX950 before the first force call");
+        return;
+    }
+    if (C.forces_are_f32)
+        k_cp_initial<float><<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.M, C.d_ci_natoms,
+            C.d_sx, C.d_sv, C.d_sf32, param->dt, param->dtforce);
+    else
+        k_cp_initial<double><<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.M, C.d_ci_natoms,
+            C.d_sx, C.d_sv, C.d_sf64, param->dt, param->dtforce);
+    MDG_LAUNCH_CHECK("cp_initial");
+}
+
+void finalIntegrateClusterGFX950(Parameter* param, Atom* atom)
+{
+    (void)atom;
+    if (!ok()) return;
+    TimedRegion tr(F_INTEGRATE);
+    int n3 = 3 * C.nci * C.M;
+    if (!(C.forces_are_f32 ? (void*)C.d_sf32 : (void*)C.d_sf64)) {
+        set_error(E_ARG, "finalIntegrateClusterGFX950 before the first force call");
+        return;
+    }
+    if (C.forces_are_f32)
+        k_cp_final<float><<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.M, C.d_ci_natoms,
+            C.d_sv, C.d_sf32, param->dtforce);
+    else
+        k_cp_final<double><<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.M, C.d_ci_natoms,
+            C.d_sv, C.d_sf64, param->dtforce);
+    MDG_LAUNCH_CHECK("cp_final");
+}
+
+double kineticSumClusterGFX950(Parameter* param, Atom* atom)
+{
+    (void)atom;
+    if (!ok()) return 0.0;
+    TimedRegion tr(F_THERMO);
+    const int nb = 512;
+    grow_n(&C.d_red, &C.cap_red, (size_t)nb, 0);
+    if (failed()) return 0.0;
+    k_cp_ksum<<<nb, BLOCK, 0, stream()>>>(C.nci * C.M, C.M, C.d_ci_natoms, C.d_sv, param->mass, C.d_red);
+    k_cp_sum_partials<<<1, 64, 0, stream()>>>(nb, C.d_red);
+    MDG_LAUNCH_CHECK("cp_ksum");
+    MDG_HIP(hipMemcpyAsync(C.h_red, C.d_red, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    return failed() ? 0.0 : C.h_red[0];
+}
+
+void requestEnergyVirialClusterGFX950(int on) { C.ev_on = on != 0; }
+
+void energyVirialClusterGFX950(double* epot, double* virial)
+{
+    double h[2] = { 0, 0 };
+    if (!failed() && C.d_ev) {
+        MDG_HIP(hipMemcpyAsync(h, C.d_ev, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        MDG_HIP(hipStreamSynchronize(stream()));
+    }
+    if (epot) *epot = h[0];
+    if (virial) *virial = h[1];
+}
+
+void clusterInfoGFX950(int* info)
+{
+    int v[11] = { C.nci, C.ncj, C.nghost, C.dummy_cj, C.nghost_atoms, C.g.mbins, C.nstencil,
+        C.maxneighs, C.M, C.N, C.natoms };
+    memcpy(info, v, sizeof(v));
+}
+
+long long downloadClusterArrayGFX950(const char* name, void* out, long long cap)
+{
+    if (!ok()) return -3;
+    hipStream_t s = stream();
+    MDG_HIP(hipStreamSynchronize(s));
+    const size_t nsl = nslots_all(), nj = (size_t)C.ncj + C.nghost;
+    const void* src  = nullptr;
+    long long n = 0, es = 4;
+    if (!strcmp(name, "slot_x")) { src = C.d_sx; n = 3 * (long long)nsl; es = 8; }
+    else if (!strcmp(name, "slot_v")) { src = C.d_sv; n = 3LL * C.nci * C.M; es = 8; }
+    else if (!strcmp(name, "slot_t")) { src = C.d_st; n = (long long)nsl; }
+    else if (!strcmp(name, "ci_natoms")) { src = C.d_ci_natoms; n = C.nci; }
+    else if (!strcmp(name, "ci_bin")) { src = C.d_ci_bin; n = C.nci; }
+    else if (!strcmp(name, "ci_bbox")) { src = C.d_ci_bbox; n = 6LL * C.nci; es = 8; }
+    else if (!strcmp(name, "cj_natoms")) { src = C.d_cj_natoms; n = (long long)nj; }
+    else if (!strcmp(name, "cj_bbox")) { src = C.d_cj_bbox; n = 6LL * nj; es = 8; }
+    else if (!strcmp(name, "ghost_map")) { src = C.d_border; n = C.nghost; }
+    else if (!strcmp(name, "ghost_pbc")) { src = C.d_pbc; n = 3LL * C.nghost; }
+    else if (!strcmp(name, "numneigh")) { src = C.d_numneigh; n = C.nci; }
+    else if (!strcmp(name, "numneigh_masked")) { src = C.d_nmasked; n = C.nci; }
+    else if (!strcmp(name, "cbin_start")) { src = C.d_jstart; n = C.g.mbins + 1; }
+    else if (!strcmp(name, "cbin_list")) { src = C.d_jlist; n = (long long)nj; }
+    else if (!strcmp(name, "atoms_x")) { src = C.d_ax; n = 3LL * C.natoms; es = 8; }
+    else if (!strcmp(name, "atoms_v")) { src = C.d_av; n = 3LL * C.natoms; es = 8; }
+    else if (!strcmp(name, "atoms_type")) { src = C.d_at; n = C.natoms; }
+    else if (!strcmp(name, "slot_f")) {
+        n = 3LL * C.nci * C.M;
+        if (n > cap) return -2;
+        if (C.forces_are_f32) {
+            float* tmp = (float*)malloc(n * sizeof(float) + 4);
+            MDG_HIP(hipMemcpy(tmp, C.d_sf32, n * sizeof(float), hipMemcpyDeviceToHost));
+            for (long long k = 0; k < n; k++)
+                ((double*)out)[k] = tmp[k];
+            free(tmp);
+        } else {
+            MDG_HIP(hipMemcpy(out, C.d_sf64, n * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return failed() ? -3 : n;
+    } else if (!strcmp(name, "neighbors")) {
+        int* nn   = (int*)malloc(sizeof(int) * (C.nci + 1));
+        int* full = (int*)malloc(sizeof(int) * ((size_t)C.nci * C.maxneighs + 1));
+        MDG_HIP(hipMemcpy(nn, C.d_numneigh, C.nci * sizeof(int), hipMemcpyDeviceToHost));
+        MDG_HIP(hipMemcpy(full, C.d_nb, (size_t)C.nci * C.maxneighs * sizeof(int),
+            hipMemcpyDeviceToHost));
+        long long k = 0;
+        for (int ci = 0; ci < C.nci && k >= 0; ci++)
+            for (int m = 0; m < nn[ci]; m++) {
+                if (k >= cap) {
+                    k = -2;
+                    break;
+                }
+                ((int*)out)[k++] = full[(size_t)ci * C.maxneighs + m];
+            }
+        free(nn);
+        free(full);
+        return failed() ? -3 : k;
+    } else {
+        return -1;
+    }
+    if (n > cap) return -2;
+    if (n) MDG_HIP(hipMemcpy(out, src, n * es, hipMemcpyDeviceToHost));
+    return failed() ? -3 : n;
+}
+
+}  // extern "C"

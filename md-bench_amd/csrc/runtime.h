@@ -11,6 +11,9 @@
 namespace mdg {
 
 // ---------------------------------------------------------------- status
+// Codes match MDG_ERR_* of mdgfx950.h (that header cannot be included next to the
+// cluster-pair structs, which reuse the reference's type names).
+enum { E_HIP = 1, E_BIN = 2, E_CAPACITY = 3, E_ARG = 4, E_NODEVICE = 5 };
 void set_error(int code, const char* fmt, ...);
 bool failed();  // true once an error is recorded; entry points then do nothing
 

@@ -140,7 +140,7 @@ void mdbEamFree(mdb_eam_tables* t)
     memset(t, 0, sizeof(*t));
 }
 
-/* Synthetic copper-like element (see mdbench_driver.h).  taper(r) = x^4/(1+x^4),
+/* Synthetic copper-like element (see mdbench_config.h).  taper(r) = x^4/(1+x^4),
    x = (r - rc)/h, vanishing with three derivatives at rc. */
 #define SYN_RE 2.556
 #define SYN_RC 4.95

@@ -28,6 +28,8 @@ class Config(C.Structure):
         ("ntypes", C.c_int),
         ("epsilon_per_type", C.POINTER(C.c_double)), ("sigma_per_type", C.POINTER(C.c_double)),
         ("eam_file", C.c_char_p), ("strict", C.c_int),
+        ("cluster_m", C.c_int), ("cluster_n", C.c_int), ("precision", C.c_int),
+        ("compensated", C.c_int),
     ]
 
 
@@ -135,6 +137,25 @@ def lib():
     L.copyDataFromGFX950Device.argtypes = [vp, vp]
     L.downloadVLStateGFX950.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_longlong]
     L.downloadBinsGFX950.argtypes = [vp, C.c_int, vp, C.c_int]
+    # cluster-pair scheme
+    L.mdbCPCreate.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.mdbCPRun.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.mdbCPSetAtoms.argtypes = [vp, _dp, _dp, _ip, C.c_int]
+    L.mdbCPDestroy.argtypes = [vp]
+    for fn in ("mdbCPParam", "mdbCPAtom", "mdbCPNeighbor", "mdbCPStats"):
+        getattr(L, fn).argtypes = [vp]
+        getattr(L, fn).restype = vp
+    L.mdbCPThermo.argtypes = [vp, C.c_double, _PD, _PD]
+    L.computeForceLJClusterGFX950.argtypes = [vp, vp, vp, vp]
+    L.computeForceLJClusterGFX950.restype = C.c_double
+    L.kineticSumClusterGFX950.argtypes = [vp, vp]
+    L.kineticSumClusterGFX950.restype = C.c_double
+    L.requestEnergyVirialClusterGFX950.argtypes = [C.c_int]
+    L.energyVirialClusterGFX950.argtypes = [_PD, _PD]
+    L.downloadClusterArrayGFX950.argtypes = [C.c_char_p, vp, C.c_longlong]
+    L.downloadClusterArrayGFX950.restype = C.c_longlong
+    L.clusterInfoGFX950.argtypes = [_ip]
+    L.setClusterPrecisionGFX950.argtypes = [C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -266,6 +287,83 @@ class VLSim:
     def energy_virial(self):
         e, w = C.c_double(), C.c_double()
         lib().energyVirialGFX950(C.byref(e), C.byref(w))
+        return e.value, w.value
+
+
+class CPSim:
+    """A cluster-pair simulation living on the GPU (include/mdbench_driver_cp.h)."""
+
+    INFO = ["nci", "ncj", "nghost", "dummy_cj", "nghost_atoms", "mbins", "nstencil",
+            "maxneighs", "M", "N", "natoms"]
+    F64 = ("slot_x", "slot_v", "slot_f", "ci_bbox", "cj_bbox", "atoms_x", "atoms_v")
+
+    def __init__(self, **kw):
+        self.cfg = default_config(**kw)
+        h = C.c_void_p()
+        rc = lib().mdbCPCreate(C.byref(self.cfg), C.byref(h))
+        if rc:
+            raise RuntimeError("mdbCPCreate failed (%d): %s" % (rc, status()[1]))
+        self.h = h
+        L = lib()
+        self.param_p = L.mdbCPParam(h)
+        self.atom_p = L.mdbCPAtom(h)
+        self.neighbor_p = L.mdbCPNeighbor(h)
+        self.stats_p = L.mdbCPStats(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mdbCPDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, nsteps, trace=False):
+        ks = np.zeros(nsteps) if trace else None
+        rc = lib().mdbCPRun(self.h, nsteps, ks.ctypes.data_as(_PD) if trace else None)
+        if rc:
+            raise RuntimeError("mdbCPRun failed (%d): %s" % (rc, status()[1]))
+        return ks
+
+    def set_atoms(self, x, v, t):
+        rc = lib().mdbCPSetAtoms(self.h, np.ascontiguousarray(x, np.float64),
+                                 np.ascontiguousarray(v, np.float64),
+                                 np.ascontiguousarray(t, np.int32), t.size)
+        if rc:
+            raise RuntimeError("mdbCPSetAtoms failed (%d): %s" % (rc, status()[1]))
+
+    def info(self):
+        arr = np.zeros(11, np.int32)
+        lib().clusterInfoGFX950(arr)
+        return dict(zip(self.INFO, arr.tolist()))
+
+    def get(self, name):
+        dt = np.float64 if name in self.F64 else np.int32
+        cap = 1 << 25
+        buf = np.zeros(cap, dt)
+        n = lib().downloadClusterArrayGFX950(name.encode(), buf.ctypes.data_as(C.c_void_p), cap)
+        if n < 0:
+            raise RuntimeError("download %s failed (%d): %s" % (name, n, status()[1]))
+        return buf[:n].copy()
+
+    def compute_force(self):
+        return lib().computeForceLJClusterGFX950(self.param_p, self.atom_p, self.neighbor_p,
+                                                 self.stats_p)
+
+    def kinetic_sum(self):
+        return lib().kineticSumClusterGFX950(self.param_p, self.atom_p)
+
+    def thermo(self, ksum):
+        t, p = C.c_double(), C.c_double()
+        lib().mdbCPThermo(self.h, ksum, C.byref(t), C.byref(p))
+        return t.value, p.value
+
+    def energy_virial(self):
+        lib().requestEnergyVirialClusterGFX950(1)
+        self.compute_force()
+        e, w = C.c_double(), C.c_double()
+        lib().energyVirialClusterGFX950(C.byref(e), C.byref(w))
+        lib().requestEnergyVirialClusterGFX950(0)
         return e.value, w.value
 
 

@@ -1,0 +1,236 @@
+"""GPU parity of the cluster-pair path against MD-Bench's own cluster-pair builds and the oracle.
+
+Bar:
+- clusters, bounding boxes, ghost clusters, cluster bins and cluster-pair lists (order included)
+  identical to the reference builds for 1x2, 4x4 and 4x8 -- at setup and after motion;
+- FP64 kernel (reference arithmetic): 1x2 full-list forces and trajectories bit-identical to
+  MD-Bench's plain-C reference kernel build; half lists within 1e-12;
+- FP32-mixed kernel: force relative RMS error <= 1e-6 against the FP64 oracle on a thermalised
+  state; energy and virial within 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import load_golden, meta, product
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {"m1n2": (1, 2), "m4n4": (4, 4), "m4n8": (4, 8)}
+FP32_RMS_TOL = 1e-6
+
+
+def _cfg(d, variant, precision=1, **extra):
+    args = [int(str(a)) for a in d["args"]]
+    M, N = SHAPES[variant]
+    kw = dict(nx=args[0], ny=args[1], nz=args[2], half_neigh=args[3], ntimes=args[4],
+              reneigh_every=args[5], cluster_m=M, cluster_n=N, precision=precision, strict=1)
+    kw.update(extra)
+    return kw
+
+
+def _live_slots(natoms, M):
+    return np.concatenate([np.arange(ci * M, ci * M + n) for ci, n in enumerate(natoms)])
+
+
+def _live_jslots(cj_natoms, N):
+    return np.concatenate([np.arange(cj * N, cj * N + n) for cj, n in enumerate(cj_natoms)])
+
+
+def _compare_structures(sim, d, tag, M, N):
+    inf = sim.info()
+    assert inf["nci"] == meta(d, tag + "_nci")
+    assert inf["ncj"] == meta(d, tag + "_ncj")
+    assert inf["nghost"] == meta(d, tag + "_nghost_clusters")
+    assert inf["dummy_cj"] == meta(d, tag + "_dummy_cj")
+    np.testing.assert_array_equal(sim.get("ci_natoms"), d[tag + "_ci_natoms"])
+    np.testing.assert_array_equal(sim.get("ci_bin"), d[tag + "_ci_bin"])
+    np.testing.assert_array_equal(sim.get("ci_bbox"), d[tag + "_ci_bbox"])
+    cjn = sim.get("cj_natoms")
+    np.testing.assert_array_equal(cjn, d[tag + "_cj_natoms"])
+    np.testing.assert_array_equal(sim.get("cj_bbox"), d[tag + "_cj_bbox"])
+    np.testing.assert_array_equal(sim.get("ghost_map"), d[tag + "_ghost_map"])
+    np.testing.assert_array_equal(sim.get("ghost_pbc"), d[tag + "_ghost_pbc"])
+    live = _live_jslots(cjn, N)  # empty slots: INF in the reference, far away here
+    sx = sim.get("slot_x").reshape(-1, 3)
+    rx = d[tag + "_slot_x"].reshape(-1, 3)
+    np.testing.assert_array_equal(sx[live], rx[live])
+    np.testing.assert_array_equal(sim.get("slot_t")[live], d[tag + "_slot_t"][live])
+    np.testing.assert_array_equal(sim.get("numneigh"), d[tag + "_numneigh"])
+    np.testing.assert_array_equal(sim.get("numneigh_masked"), d[tag + "_numneigh_masked"])
+    np.testing.assert_array_equal(sim.get("neighbors"), d[tag + "_neighbors"])
+
+
+CASES = [(v, s) for v in SHAPES for s in ("4", "4_half", "5x6x7")]
+
+
+@pytest.mark.parametrize("variant,size", CASES)
+def test_setup_structures_match_reference(variant, size):
+    print("case", variant, size, flush=True)
+    d = load_golden("cp_%s_%s" % (variant, size))
+    M, N = SHAPES[variant]
+    sim = product().CPSim(**_cfg(d, variant))
+    try:
+        _compare_structures(sim, d, "s0", M, N)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("variant,size", CASES)
+def test_rebuild_after_motion_matches_reference(variant, size):
+    print("case", variant, size, flush=True)
+    d = load_golden("cp_%s_%s" % (variant, size))
+    M, N = SHAPES[variant]
+    sim = product().CPSim(**_cfg(d, variant))
+    try:
+        sim.set_atoms(d["pre1_x"], d["pre1_v"], d["pre1_type"])
+        _compare_structures(sim, d, "r1", M, N)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("size", ["4", "4_half", "5x6x7"])
+def test_fp64_m1n2_force_and_trajectory_match_reference(size):
+    print("case m1n2", size, flush=True)
+    d = load_golden("cp_m1n2_%s" % size)
+    args = [int(str(a)) for a in d["args"]]
+    half = args[3]
+    sim = product().CPSim(**_cfg(d, "m1n2"))
+    try:
+        nci = meta(d, "s0_nci")
+        f = sim.get("slot_f")
+        if half:
+            assert np.max(np.abs(f - d["s0_slot_f"][:3 * nci])) < 1e-12
+        else:
+            np.testing.assert_array_equal(f, d["s0_slot_f"][:3 * nci])
+        ks = sim.run(args[4], trace=True)
+        np.testing.assert_allclose(ks, d["traj_ksum"][1:], rtol=1e-12)
+        cjn = sim.get("cj_natoms")
+        live = _live_jslots(cjn[:sim.info()["ncj"]], 2)
+        sx = sim.get("slot_x").reshape(-1, 3)[live]
+        rx = d["end_slot_x"].reshape(-1, 3)[live]
+        if half:
+            np.testing.assert_allclose(sx, rx, rtol=0, atol=1e-10)
+        else:
+            np.testing.assert_array_equal(sx, rx)
+    finally:
+        sim.close()
+
+
+def _thermalised_atoms(nx, steps=60):
+    """A liquid-like state from the Verlet-list oracle (FP64), wrapped into the box."""
+    P = product()
+    x0, v0, t0 = P.create_lattice(nx=nx, ny=nx, nz=nx)
+    lat = (4 / 0.8442) ** (1 / 3)
+    one = np.ones(16)
+    x, v, f, ks = ob.vl_run(nx, nx, nx, lat, x0, v0, t0, 4, one, one, one * 6.25, one * 7.84,
+                            2.8, 0, steps, 20, 0.005, 0.0025, 1.0)
+    ob.lib().orc_update_atoms_pbc(x.size // 3, x, nx * lat, nx * lat, nx * lat)
+    return x, v, t0, lat
+
+
+def _oracle_forces(sim, M, N, half, lat, nx):
+    """FP64 oracle forces on exactly the device's slots (same clusters, same lists)."""
+    ax, av, at = sim.get("atoms_x"), sim.get("atoms_v"), sim.get("atoms_type")
+    cp = ob.CP(M, N, (nx * lat,) * 3, 2.8, ax, av, at)
+    cp.reneighbour(half)
+    f, e, w = cp.force(half)
+    return cp, f, e, w
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+@pytest.mark.parametrize("half", [0, 1])
+@pytest.mark.parametrize("comp", [0, 1])
+def test_fp32_force_error_vs_fp64_oracle(shape, half, comp):
+    M, N = shape
+    nx = 8
+    print("fp32", shape, "half", half, "comp", comp, flush=True)
+    x, v, t, lat = _thermalised_atoms(nx)
+    P = product()
+    sim = P.CPSim(nx=nx, ny=nx, nz=nx, half_neigh=half, cluster_m=M, cluster_n=N, precision=0,
+                  compensated=comp, ntimes=0)
+    try:
+        sim.set_atoms(x, v, t)
+        cp, f64, e64, w64 = _oracle_forces(sim, M, N, half, lat, nx)
+        # the device and the oracle built the same clusters from the same atoms
+        np.testing.assert_array_equal(sim.get("neighbors"), cp.get("neighbors"))
+        nat = sim.get("ci_natoms")
+        live = _live_slots(nat, M)
+        f32 = sim.get("slot_f").reshape(-1, 3)[live]
+        fr = f64.reshape(-1, 3)[live]
+        rms = np.sqrt(((f32 - fr) ** 2).sum() / (fr ** 2).sum())
+        print("relative RMS force error %.3e" % rms, flush=True)
+        assert rms <= FP32_RMS_TOL
+        e, w = sim.energy_virial()
+        assert abs(e - e64) <= 1e-6 * abs(e64)
+        assert abs(w - w64) <= 1e-6 * abs(w64)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("half", [0, 1])
+def test_fp64_kernel_matches_oracle_on_thermalised_state(half):
+    M, N, nx = 4, 4, 8
+    x, v, t, lat = _thermalised_atoms(nx)
+    sim = product().CPSim(nx=nx, ny=nx, nz=nx, half_neigh=half, cluster_m=M, cluster_n=N,
+                          precision=1, ntimes=0)
+    try:
+        sim.set_atoms(x, v, t)
+        cp, f64, e64, w64 = _oracle_forces(sim, M, N, half, lat, nx)
+        nat = sim.get("ci_natoms")
+        live = _live_slots(nat, M)
+        fd = sim.get("slot_f").reshape(-1, 3)[live]
+        fr = f64.reshape(-1, 3)[live]
+        if half:
+            assert np.linalg.norm(fd - fr) / np.linalg.norm(fr) < 1e-12
+        else:
+            np.testing.assert_array_equal(fd, fr)
+    finally:
+        sim.close()
+
+
+def test_fp32_trajectory_tracks_fp64_and_conserves_energy():
+    """16^3 cells, 200 steps: FP32-mixed and FP64 kernels give the same thermodynamics."""
+    P = product()
+    res = {}
+    for prec in (0, 1):
+        sim = P.CPSim(nx=16, ny=16, nz=16, precision=prec, ntimes=200)
+        try:
+            ks = sim.run(200, trace=True)
+            res[prec] = sim.thermo(ks[-1])[0]
+        finally:
+            sim.close()
+    print("T(200) fp32 %.7e fp64 %.7e" % (res[0], res[1]), flush=True)
+    assert abs(res[0] - res[1]) / res[1] < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+def test_full_size_fp32_vs_fp64_kernel(shape):
+    """64^3 cells (1,048,576 atoms): the FP32-mixed kernel against the device FP64 kernel (the
+    latter bit-identical to the oracle on small systems) on the same equilibrated state.  The
+    relative RMS error is measured after 200 steps: right after the lattice start the net forces
+    are small against the pair forces and the relative measure is ill-conditioned."""
+    M, N = shape
+    P = product()
+    sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=1, ntimes=200)
+    try:
+        sim.run(200)  # step 200 rebuilds: atoms_* hold the current positions
+        state = (sim.get("atoms_x"), sim.get("atoms_v"), sim.get("atoms_type"))
+    finally:
+        sim.close()
+    forces = {}
+    for prec in (1, 0):
+        sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=prec, ntimes=0)
+        try:
+            sim.set_atoms(*state)
+            forces[prec] = sim.get("slot_f")
+            nat = sim.get("ci_natoms")
+        finally:
+            sim.close()
+    live = _live_slots(nat, M)
+    a, b = forces[0].reshape(-1, 3)[live], forces[1].reshape(-1, 3)[live]
+    rms = np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+    print("1M atoms %dx%d: relative RMS force error %.3e, %d atoms" % (M, N, rms, live.size),
+          flush=True)
+    assert live.size == 4 * 64 ** 3
+    assert rms <= FP32_RMS_TOL
