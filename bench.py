@@ -13,6 +13,7 @@ Workloads (seeded fcc lattices, synthetic velocities):
              rho* 0.8442, r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
   cp-lj-64-4x8, cp-lj-64-half: the same with j-clusters of 8 / a half list.
   vl-lj-64   Verlet-list LJ, FP64, full list, same lattice.
+  hbm-copy   a plain 256 MiB streaming kernel: the scale factor for the HBM byte counters.
 
 `roofline` covers the dominant kernel (the force kernel): algorithmic bytes per launch over its
 average launch time, both measured inside the timed region with HIP events on the library stream.
@@ -200,6 +201,38 @@ def bench_cp(args, mod, N=4, half=0):
         sim.close()
 
 
+def bench_copy(args, mod):
+    """Plain streaming kernel of known size, the scale for HBM counters: each launch of
+    torch.mul(src, 1.0, out=dst) reads and writes 256 MiB (2^26 FP32 values each way)."""
+    import torch
+    n = 1 << 26
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(max(args.warmup, 1)):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        torch.mul(src, 1.0, out=dst)
+    ev1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    nbytes = 2.0 * 4 * n
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    res = {"metric": "HBM copy bandwidth (counter scale)", "value": gbs, "unit": "GB/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * (t1 - t0) / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "hbm-copy: 2^26 FP32 read + written per launch",
+                      "bytes_per_launch": nbytes},
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "traffic": nbytes}}
+    return res, None
+
+
 import numpy as np  # noqa: E402
 
 WORKLOADS = {
@@ -207,6 +240,7 @@ WORKLOADS = {
     "cp-lj-64-4x8": lambda a, m: bench_cp(a, m, N=8),
     "cp-lj-64-half": lambda a, m: bench_cp(a, m, half=1),
     "vl-lj-64": bench_vl,
+    "hbm-copy": bench_copy,
 }
 DEFAULT = "cp-lj-64"
 
@@ -226,7 +260,7 @@ def main():
     mod = binding()
     res, nx = WORKLOADS[args.workload](args, mod)
     cb = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and nx is not None:
         try:
             cb = (cpu_baseline_vl if args.workload.startswith("vl") else cpu_baseline_cp)(nx, 20)
         except Exception as exc:  # the GPU figure stands on its own

@@ -29,6 +29,15 @@ typedef struct {
     int cluster_m, cluster_n; /* i- and j-cluster sizes: 4x4 (default), 4x8, 1x2 */
     int precision;     /* 0 = FP32 pair math with FP64 state (default), 1 = FP64 */
     int compensated;   /* 1 = Kahan-compensated FP32 force sums */
+    /* explicit type-pair tables (LJ_COMB_RULE none, src/common/parameter.h:60): ntypes^2
+       epsilon, sigma and force cutoff per pair; NULL = geometric mixing of per-type values */
+    const double* pair_epsilon;
+    const double* pair_sigma;
+    const double* pair_cutoff;
+    /* > 0: types drawn per atom from a Park-Miller stream seeded with type_seed, type 1 with
+       this probability, type 0 otherwise (replaces rand() % ntypes) */
+    double type1_fraction;
+    int type_seed;
 } mdb_config;
 
 /* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */

@@ -11,8 +11,9 @@
 //                   src/clusterpair/force_lj.c:22-189); full lists are bit-identical to it.
 //   k_cp_force_f32  one wavefront per i-cluster of 4 atoms: lane = (i-atom 0..3) x (j-slot
 //                   0..15), i.e. four 4-atom or two 8-atom j-clusters per iteration.  The pair
-//                   displacement is formed in FP64 from the FP64 positions and rounded once to
-//                   FP32; everything after runs in FP32 with a Newton-refined v_rcp_f32.  The
+//                   displacement, r^2 and the cutoff test are FP64 (the FP64 kernel's pair set);
+//                   r^2 and the displacement are rounded once to FP32 and everything after runs
+//                   in FP32 with a Newton-refined v_rcp_f32.  The
 //                   full list is atomic-free; the half list reduces j-forces over the four
 //                   i-lanes with cross-lane shuffles and issues one atomic per j-atom.
 #include <hip/hip_runtime.h>
@@ -48,6 +49,7 @@ struct Tab64 {
 };
 struct Tab32 {
     float eps48[MAXT2], eps4[MAXT2], sig6[MAXT2], cut2[MAXT2];
+    double cut2d[MAXT2];  // cutoffs for the FP64 inclusion test
 };
 
 struct CPState {
@@ -749,13 +751,14 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
     float* __restrict__ f, double* __restrict__ ev)
 {
     constexpr int CPI = 16 / N;  // j-clusters per iteration
-    __shared__ float s_tab[4][MAXT2];
+    __shared__ float s_tab[3][MAXT2];
+    __shared__ double s_cut[MAXT2];
     if (!UNIFORM) {
         for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
             s_tab[0][k] = tab.eps48[k];
             s_tab[1][k] = tab.eps4[k];
             s_tab[2][k] = tab.sig6[k];
-            s_tab[3][k] = tab.cut2[k];
+            s_cut[k]    = tab.cut2d[k];
         }
         __syncthreads();
     }
@@ -776,7 +779,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
     const int ti     = UNIFORM ? 0 : st[si];
     const int nn     = numneigh[ci];
     const int* lst   = nb + (size_t)ci * maxneighs;
-    const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0], cut2u = tab.cut2[0];
+    const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
+    const double cut2du = tab.cut2d[0];
     float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     float e = 0.f, w = 0.f;
     for (int k0 = 0; k0 < nn; k0 += CPI) {
@@ -784,20 +788,27 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
         const bool valid = k < nn;
         const int cj     = valid ? lst[k] : ci_cj0;
         const int sj     = cj * N + jj;
-        const float dx   = (float)(xi - sx[3 * sj]);
-        const float dy   = (float)(yi - sx[3 * sj + 1]);
-        const float dz   = (float)(zi - sx[3 * sj + 2]);
-        const float rsq  = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u, cut2 = cut2u;
+        // Displacement and r^2 in FP64, evaluated as the FP64 reference kernel evaluates them,
+        // so that the set of pairs inside the cutoff is exactly the reference's (a pair that
+        // crossed r_c on an FP32 rounding would change a force by the full F(r_c)).  Then one
+        // rounding each of r^2 and the displacement to FP32 for the force arithmetic.
+        const double dxd = xi - sx[3 * sj];
+        const double dyd = yi - sx[3 * sj + 1];
+        const double dzd = zi - sx[3 * sj + 2];
+        const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+        const float rsq  = (float)rsqd;
+        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u;
+        double cut2d = cut2du;
         if (!UNIFORM) {
             const int tij = ti * ntypes + st[sj];
             eps48         = s_tab[0][tij];
             eps4          = s_tab[1][tij];
             sig6          = s_tab[2][tij];
-            cut2          = s_tab[3][tij];
+            cut2d         = s_cut[tij];
         }
         const bool excl = (cj == ci_cj0) && (HALF ? !(si < sj) : (si == sj));
-        const bool in   = valid && ilive && !excl && rsq < cut2;
+        const bool in   = valid && ilive && !excl && rsqd < (double)cut2d;
         float r2i       = __builtin_amdgcn_rcpf(rsq);
         r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
         const float sr6 = r2i * r2i * r2i * sig6;
@@ -1014,6 +1025,7 @@ void initDeviceClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
         C.t32.eps4[k]   = (float)(4.0 * atom->epsilon[k]);
         C.t32.sig6[k]   = (float)atom->sigma6[k];
         C.t32.cut2[k]   = (float)atom->cutforcesq[k];
+        C.t32.cut2d[k]  = atom->cutforcesq[k];
         if (atom->epsilon[k] != atom->epsilon[0] || atom->sigma6[k] != atom->sigma6[0] ||
             atom->cutforcesq[k] != atom->cutforcesq[0])
             C.uniform = false;

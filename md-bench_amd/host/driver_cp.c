@@ -111,7 +111,14 @@ int mdbCPCreate(const mdb_config* cfg, mdb_cp** out)
         a->vz[i] = v[3 * i + 2];
     }
     free(v);
-    /* type-pair tables: geometric mixing of per-type parameters (parameter.c:31-70) */
+    if (cfg->type1_fraction > 0.0) {
+        /* seeded binary mixture: one Park-Miller stream in atom order */
+        int seed = cfg->type_seed > 0 ? cfg->type_seed : 1;
+        for (int i = 0; i < natom; i++)
+            a->type[i] = mdbMyrandom(&seed) < cfg->type1_fraction ? 1 : 0;
+    }
+    /* type-pair tables: explicit pairs, or geometric mixing of per-type parameters
+       (parameter.c:31-70) */
     int nt = a->ntypes;
     a->epsilon    = malloc(sizeof(double) * nt * nt);
     a->sigma6     = malloc(sizeof(double) * nt * nt);
@@ -127,6 +134,13 @@ int mdbCPCreate(const mdb_config* cfg, mdb_cp** out)
             a->sigma6[i * nt + j]     = (si * si * si) * (sj * sj * sj);
             a->cutforcesq[i * nt + j] = p->cutforce * p->cutforce;
             a->cutneighsq[i * nt + j] = p->cutneigh * p->cutneigh;
+            if (cfg->pair_epsilon && cfg->pair_sigma) {
+                double e = cfg->pair_epsilon[i * nt + j], sg = cfg->pair_sigma[i * nt + j];
+                double c = cfg->pair_cutoff ? cfg->pair_cutoff[i * nt + j] : p->cutforce;
+                a->epsilon[i * nt + j]    = e;
+                a->sigma6[i * nt + j]     = sg * sg * sg * sg * sg * sg;
+                a->cutforcesq[i * nt + j] = c * c;
+            }
         }
     /* thermo scaling (thermo.c:35-58, LJ units) */
     s->dof_boltz = (natom * 3 - 3);

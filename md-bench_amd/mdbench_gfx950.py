@@ -30,6 +30,9 @@ class Config(C.Structure):
         ("eam_file", C.c_char_p), ("strict", C.c_int),
         ("cluster_m", C.c_int), ("cluster_n", C.c_int), ("precision", C.c_int),
         ("compensated", C.c_int),
+        ("pair_epsilon", C.POINTER(C.c_double)), ("pair_sigma", C.POINTER(C.c_double)),
+        ("pair_cutoff", C.POINTER(C.c_double)), ("type1_fraction", C.c_double),
+        ("type_seed", C.c_int),
     ]
 
 
@@ -170,7 +173,8 @@ def default_config(**kw):
     lib().mdbDefaultConfig(C.byref(cfg))
     keep = []
     for k, v in kw.items():
-        if k in ("epsilon_per_type", "sigma_per_type") and v is not None:
+        if k in ("epsilon_per_type", "sigma_per_type", "pair_epsilon", "pair_sigma",
+                 "pair_cutoff") and v is not None:
             arr = (C.c_double * len(v))(*v)
             keep.append(arr)
             setattr(cfg, k, C.cast(arr, C.POINTER(C.c_double)))

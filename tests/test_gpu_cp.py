@@ -234,3 +234,62 @@ def test_full_size_fp32_vs_fp64_kernel(shape):
           flush=True)
     assert live.size == 4 * 64 ** 3
     assert rms <= FP32_RMS_TOL
+
+
+KA = dict(ntypes=2, pair_epsilon=[1.0, 1.5, 1.5, 0.5], pair_sigma=[1.0, 0.8, 0.8, 0.88],
+          pair_cutoff=[2.5, 2.0, 2.0, 2.2], rho=1.2, temp=0.8, type1_fraction=0.2, type_seed=4711)
+
+
+def _vl_oracle_forces(x, t, nx, lat, eps, sig6, cutf, ntypes, cutneigh=2.8):
+    """FP64 Verlet-list oracle forces (type-pair tables) on atoms x (wrapped into the box)."""
+    nl = x.size // 3
+    prd = (nx * lat,) * 3
+    g = ob.vl_grid(nx, nx, nx, lat, cutneigh)
+    bmap, pbc = ob.setup_pbc(nl, x, prd, cutneigh)
+    xall = np.zeros(3 * (nl + bmap.size))
+    xall[:3 * nl] = x
+    xall = ob.update_pbc(nl, xall, bmap, pbc, prd)
+    tall = np.concatenate([t, t[bmap]]).astype(np.int32)
+    cn = np.full(ntypes * ntypes, cutneigh ** 2)
+    nn, nb = ob.build_neighbor(g, nl, xall, tall, ntypes, cn, 0)
+    f, e, w = ob.force_lj(nl, xall, tall, ntypes, eps, sig6, cutf, nn, nb, 0)
+    return f[:3 * nl].reshape(-1, 3), e, w
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+def test_kob_andersen_mixture_fp32_vs_fp64_oracle(shape):
+    """Binary LJ mixture (eps_AB 1.5, sigma_AB 0.8, eps_BB 0.5, sigma_BB 0.88, per-pair cutoff
+    2.5 sigma_ab, rho 1.2, 80/20 seeded types) on a 16^3 sub-box: the FP32-mixed kernel with its
+    type-pair table in LDS against the FP64 Verlet-list oracle on the same positions."""
+    M, N = shape
+    nx = 16
+    P = product()
+    sim = P.CPSim(nx=nx, ny=nx, nz=nx, cluster_m=M, cluster_n=N, precision=0, ntimes=100, **KA)
+    try:
+        sim.run(100)
+        sim.compute_force()  # forces on the current positions
+        x, t = sim.get("atoms_x"), sim.get("atoms_type")
+        frac = t.mean()
+        # forces of the live slots, mapped back to the atoms in cluster order
+        nat = sim.get("ci_natoms")
+        live = _live_slots(nat, M)
+        sx = sim.get("slot_x").reshape(-1, 3)[live]
+        st = sim.get("slot_t")[live]
+        f32 = sim.get("slot_f").reshape(-1, 3)[live]
+        e32, w32 = sim.energy_virial()
+    finally:
+        sim.close()
+    assert 0.17 < frac < 0.23
+    lat = (4 / 1.2) ** (1 / 3)
+    L = nx * lat
+    xw = np.mod(sx, L).reshape(-1)  # slot positions drift out of the box between rebuilds
+    eps = np.array(KA["pair_epsilon"])
+    sig6 = np.array(KA["pair_sigma"]) ** 6
+    cutf = np.array(KA["pair_cutoff"]) ** 2
+    fr, e64, w64 = _vl_oracle_forces(xw, st, nx, lat, eps, sig6, cutf, 2)
+    rms = np.sqrt(((f32 - fr) ** 2).sum() / (fr ** 2).sum())
+    print("KA %dx%d: relative RMS force error %.3e, energy %.6e vs %.6e" % (M, N, rms, e32, e64),
+          flush=True)
+    assert rms <= FP32_RMS_TOL
+    assert abs(e32 - e64) <= 1e-6 * abs(e64)
+    assert abs(w32 - w64) <= 1e-6 * abs(w64)
