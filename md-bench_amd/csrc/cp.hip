@@ -258,6 +258,32 @@ __global__ void __launch_bounds__(64) k_cp_bin_sort(const int* __restrict__ star
         s_z[p]   = ax[3 * s_tmp[p] + 2];
     }
     __syncthreads();
+    // With all z distinct the selection sort's result is the unique sorted order: rank by z.
+    // Only a column with equal z values (a perfect lattice) replays the selection sort, whose
+    // swaps decide the order of ties.
+    bool tie = false;
+    for (int p = lane; p < c; p += 64) {
+        const double zp = s_z[p];
+        int r = 0;
+        for (int q = 0; q < c; q++) {
+            const double zq = s_z[q];
+            r += zq < zp;
+            tie |= (zq == zp) && (q != p);
+        }
+        s_tmp[r] = s_idx[p];
+    }
+    if (!__any(tie)) {
+        __syncthreads();
+        for (int p = lane; p < c; p += 64)
+            atoms[s0 + p] = s_tmp[p];
+        if (lane == 0) {
+            int nc = (c + M - 1) / M;
+            if (N > M && (nc % 2)) nc++;
+            ncl[b] = nc;
+        }
+        return;
+    }
+    __syncthreads();
     for (int a = 0; a < c; a++) {
         double bz = INFINITY;
         int bp    = INT_MAX;
@@ -547,15 +573,48 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
 }
 
 // ---------------------------------------------------------------- buildNeighborCPU (neighbor.c:282-766)
-// One wavefront per i-cluster.  Candidates (stencil columns in order, clusters in column order)
-// are tested 64 at a time; ballots give every accepted j-cluster its place in the list.
+// Largest z-extent of any j-cluster (local or ghost): bounds bbmaxz from bbminz in the column
+// search below.  Positive doubles order like their bit patterns.
+__global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
+    unsigned long long* __restrict__ hmax)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double h = 0.0;
+    if (k < n) h = fmax(cj_bbox[6 * k + 5] - cj_bbox[6 * k + 4], 0.0);
+    for (int o = 32; o > 0; o >>= 1)
+        h = fmax(h, __shfl_xor(h, o, 64));
+    if ((threadIdx.x & 63) == 0 && h > 0.0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
+}
+
+// First position m in [lo, hi) of a bbminz-sorted column whose key is >= v (strict: > v).
+template <bool STRICT>
+__device__ __forceinline__ int col_search(const int* __restrict__ jlist,
+    const double* __restrict__ cj_bbox, int lo, int hi, double v)
+{
+    while (lo < hi) {
+        int mid  = (lo + hi) >> 1;
+        double k = cj_bbox[6 * jlist[mid] + 4];
+        if (STRICT ? (k <= v) : (k < v)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One wavefront per i-cluster.  Each stencil column is sorted by bbminz, so the j-clusters
+// whose z-gap to the i-cluster alone exceeds the list cutoff form a prefix and a suffix of the
+// column; two binary searches cut them off (a j-cluster's bbmaxz is at most its bbminz plus the
+// largest cluster height).  Dropped candidates are exactly ones the bounding-box test would
+// reject, so the list is the reference's, in the reference's order (stencil columns in order,
+// clusters in column order).  The remaining candidates of up to 64 columns are flattened and
+// tested 64 at a time; ballots give every accepted j-cluster its place in the list.
 template <int M, int N>
 __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ cj_bbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist, const double* __restrict__ sx,
-    double cutneighsq, double rbb_sq, int maxneighs, int* __restrict__ nb,
-    int* __restrict__ numneigh, int* __restrict__ nmasked, int* __restrict__ status)
+    double cutneighsq, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
+    int maxneighs, int* __restrict__ nb, int* __restrict__ numneigh, int* __restrict__ nmasked,
+    int* __restrict__ status)
 {
     __shared__ int s_st[1024];
     __shared__ int s_mpos[BLOCK / 64][4];
@@ -571,21 +630,51 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
 #pragma unroll
     for (int k = 0; k < 6; k++)
         ib[k] = ci_bbox[6 * ci + k];
+    // search window in bbminz; the margin only widens it
+    const double R    = sqrt(cutneighsq) * (1.0 + 1e-12) + 1e-9;
+    const double hmax = __longlong_as_double((long long)*hmax_bits);
+    const double zlo  = ib[4] - R - hmax * (1.0 + 1e-12) - 1e-9;
+    const double zhi  = ib[5] + R;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int* out = nb + (size_t)ci * maxneighs;
     int n = 0, nm = 0;
-    for (int s = 0; s < nstencil; s++) {
-        int jb = ibin + s_st[s];
-        if (jb < 0 || jb >= mbins) {
-            if (lane == 0) atomicOr(status, ST_BIN_RANGE);
-            continue;
+    for (int s0 = 0; s0 < nstencil; s0 += 64) {
+        const int s = s0 + lane;
+        int lo = 0, cnt = 0;
+        if (s < nstencil) {
+            const int jb = ibin + s_st[s];
+            if (jb < 0 || jb >= mbins) {
+                atomicOr(status, ST_BIN_RANGE);
+            } else {
+                const int beg = jstart[jb], end = jstart[jb + 1];
+                lo           = col_search<false>(jlist, cj_bbox, beg, end, zlo);
+                const int hi = col_search<true>(jlist, cj_bbox, lo, end, zhi);
+                cnt          = hi - lo;
+            }
         }
-        const int beg = jstart[jb], end = jstart[jb + 1];
-        for (int c0 = beg; c0 < end; c0 += 64) {
-            const int m = c0 + lane;
+        // exclusive scan of the per-column counts
+        int inc = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += v;
+        }
+        const int total = __shfl(inc, 63, 64);
+        const int pre   = inc - cnt;
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            const int t = t0 + lane;
+            // largest column c with pre[c] <= t: that column holds candidate t (columns with no
+            // candidates share their successor's pre and are skipped over)
+            int c = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int pc = __shfl(pre, c + step, 64);
+                if (pc <= t) c += step;
+            }
+            const int m = __shfl(lo, c, 64) + (t - __shfl(pre, c, 64));
             bool pass = false, masked = false;
             int cj    = -1;
-            if (m < end) {
+            if (t < total) {
                 cj = jlist[m];
                 if (!(half && ci_cj0 > cj)) {
                     const double* jbb = &cj_bbox[6 * cj];
@@ -1356,6 +1445,12 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     hipStream_t s = stream();
     const int nci = C.nci;
     if (neighbor->maxneighs > C.maxneighs) C.maxneighs = neighbor->maxneighs;
+    // largest j-cluster height (status words 12-13 hold it as a 64-bit pattern)
+    unsigned long long* d_hmax = reinterpret_cast<unsigned long long*>(C.d_status + 12);
+    fill_i32(C.d_status + 12, 0, 2);
+    const int ncj_all = C.ncj + C.nghost;
+    if (ncj_all > 0)
+        k_cp_jheight<<<blocks_for(ncj_all, BLOCK), BLOCK, 0, s>>>(ncj_all, C.d_cj_bbox, d_hmax);
     for (int attempt = 0; attempt < 8; attempt++) {
         grow_n(&C.d_nb, &C.cap_nb, (size_t)nci * C.maxneighs + 1, 0);
         grow_n(&C.d_numneigh, &C.cap_numneigh, (size_t)nci + 1, 0);
@@ -1366,7 +1461,7 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
         C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_cj_bbox, C.d_jstart,      \
-        C.d_jlist, C.d_sx, C.cutneighsq, C.rbb_sq, C.maxneighs, C.d_nb, C.d_numneigh,             \
+        C.d_jlist, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs, C.d_nb, C.d_numneigh,     \
         C.d_nmasked, C.d_status)
         if (C.M == 4 && C.N == 4) MDG_NB(4, 4);
         else if (C.M == 4 && C.N == 8) MDG_NB(4, 8);
