@@ -18,6 +18,7 @@
 //                   i-lanes with cross-lane shuffles and issues one atomic per j-atom.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -578,12 +579,19 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
 __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
     unsigned long long* __restrict__ hmax)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_h[BLOCK / 64];
     double h = 0.0;
-    if (k < n) h = fmax(cj_bbox[6 * k + 5] - cj_bbox[6 * k + 4], 0.0);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        h = fmax(h, cj_bbox[6 * k + 5] - cj_bbox[6 * k + 4]);
     for (int o = 32; o > 0; o >>= 1)
         h = fmax(h, __shfl_xor(h, o, 64));
-    if ((threadIdx.x & 63) == 0 && h > 0.0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
+    if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; w++)
+            h = fmax(h, s_h[w]);
+        if (h > 0.0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
+    }
 }
 
 // First position m in [lo, hi) of a bbminz-sorted column whose key is >= v (strict: > v).
@@ -856,7 +864,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
     // multiple of 8 blocks, which makes the remapping a permutation.
     const int per = gridDim.x >> 3;
     const int lb  = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    const int ci  = lb * (BLOCK / 64) + (threadIdx.x >> 6);
+    // wave-uniform values live in scalar registers, so loop control stays scalar
+    const int ci = __builtin_amdgcn_readfirstlane(lb * (BLOCK / 64) + (threadIdx.x >> 6));
     if (ci >= nci) return;
     const int lane = threadIdx.x & 63;
     const int a = lane & 3, s = lane >> 2;
@@ -866,37 +875,57 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
     const double xi = sx[3 * si], yi = sx[3 * si + 1], zi = sx[3 * si + 2];
     const int ci_cj0 = (N == 4) ? ci : ci >> 1;
     const int ti     = UNIFORM ? 0 : st[si];
-    const int nn     = numneigh[ci];
+    const int nn     = __builtin_amdgcn_readfirstlane(numneigh[ci]);
     const int* lst   = nb + (size_t)ci * maxneighs;
     const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
     const double cut2du = tab.cut2d[0];
     float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     float e = 0.f, w = 0.f;
+    // The list of this i-cluster is held in registers, 64 entries at a time (one per lane), and
+    // handed out with cross-lane reads; entries past the end read as the i-cluster's own
+    // j-cluster, a valid address whose pairs are masked.  The positions of the next iteration
+    // are loaded before the current ones are used.
+    int ent = lane < nn ? lst[lane] : ci_cj0;
+    int cj  = __shfl(ent, g, 64);
+    int sj  = cj * N + jj;
+    double xj = sx[3 * sj], yj = sx[3 * sj + 1], zj = sx[3 * sj + 2];
     for (int k0 = 0; k0 < nn; k0 += CPI) {
         const int k      = k0 + g;
         const bool valid = k < nn;
-        const int cj     = valid ? lst[k] : ci_cj0;
-        const int sj     = cj * N + jj;
+        const int ccj = cj, csj = sj;
+        const double cxj = xj, cyj = yj, czj = zj;
+        const int kn = k0 + CPI;
+        if (kn < nn) {
+            if ((kn & 63) == 0) ent = kn + lane < nn ? lst[kn + lane] : ci_cj0;
+            const int kk = kn + g;
+            // the cross-lane read runs in every lane: a lane outside a branch supplies no data
+            const int nxt = __shfl(ent, kk & 63, 64);
+            cj            = kk < nn ? nxt : ci_cj0;
+            sj = cj * N + jj;
+            xj = sx[3 * sj];
+            yj = sx[3 * sj + 1];
+            zj = sx[3 * sj + 2];
+        }
         // Displacement and r^2 in FP64, evaluated as the FP64 reference kernel evaluates them,
         // so that the set of pairs inside the cutoff is exactly the reference's (a pair that
         // crossed r_c on an FP32 rounding would change a force by the full F(r_c)).  Then one
         // rounding each of r^2 and the displacement to FP32 for the force arithmetic.
-        const double dxd = xi - sx[3 * sj];
-        const double dyd = yi - sx[3 * sj + 1];
-        const double dzd = zi - sx[3 * sj + 2];
+        const double dxd = xi - cxj;
+        const double dyd = yi - cyj;
+        const double dzd = zi - czj;
         const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
         const float rsq  = (float)rsqd;
         const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
         float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u;
         double cut2d = cut2du;
         if (!UNIFORM) {
-            const int tij = ti * ntypes + st[sj];
+            const int tij = ti * ntypes + st[csj];
             eps48         = s_tab[0][tij];
             eps4          = s_tab[1][tij];
             sig6          = s_tab[2][tij];
             cut2d         = s_cut[tij];
         }
-        const bool excl = (cj == ci_cj0) && (HALF ? !(si < sj) : (si == sj));
+        const bool excl = (ccj == ci_cj0) && (HALF ? !(si < csj) : (si == csj));
         const bool in   = valid && ilive && !excl && rsqd < (double)cut2d;
         float r2i       = __builtin_amdgcn_rcpf(rsq);
         r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
@@ -913,7 +942,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
             fz = fmaf(dz, fpair, fz);
         }
         if (EV) {
-            const float wgt = (HALF && cj < ncj) ? 1.0f : 0.5f;
+            const float wgt = (HALF && ccj < ncj) ? 1.0f : 0.5f;
             e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
             w += wgt * rsq * fpair;
         }
@@ -926,10 +955,10 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
             jx += __shfl_xor(jx, 2, 64);
             jy += __shfl_xor(jy, 2, 64);
             jz += __shfl_xor(jz, 2, 64);
-            if (a == 0 && valid && cj < ncj && (jx != 0.f || jy != 0.f || jz != 0.f)) {
-                atomicAdd(&f[3 * sj], jx);
-                atomicAdd(&f[3 * sj + 1], jy);
-                atomicAdd(&f[3 * sj + 2], jz);
+            if (a == 0 && valid && ccj < ncj && (jx != 0.f || jy != 0.f || jz != 0.f)) {
+                atomicAdd(&f[3 * csj], jx);
+                atomicAdd(&f[3 * csj + 1], jy);
+                atomicAdd(&f[3 * csj + 2], jz);
             }
         }
     }
@@ -1450,7 +1479,8 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     fill_i32(C.d_status + 12, 0, 2);
     const int ncj_all = C.ncj + C.nghost;
     if (ncj_all > 0)
-        k_cp_jheight<<<blocks_for(ncj_all, BLOCK), BLOCK, 0, s>>>(ncj_all, C.d_cj_bbox, d_hmax);
+        k_cp_jheight<<<std::min(blocks_for(ncj_all, BLOCK), 512u), BLOCK, 0, s>>>(ncj_all,
+            C.d_cj_bbox, d_hmax);
     for (int attempt = 0; attempt < 8; attempt++) {
         grow_n(&C.d_nb, &C.cap_nb, (size_t)nci * C.maxneighs + 1, 0);
         grow_n(&C.d_numneigh, &C.cap_numneigh, (size_t)nci + 1, 0);

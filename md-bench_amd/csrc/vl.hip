@@ -54,6 +54,9 @@ struct State {
     size_t cap_bins = 0, cap_binstart = 0;
     int* d_binatoms = nullptr;
     size_t cap_binatoms = 0;
+    // local atoms in bin order (the list build's thread order)
+    int *d_lflag = nullptr, *d_lpos = nullptr, *d_order = nullptr;
+    size_t cap_lflag = 0, cap_lpos = 0, cap_order = 0;
     bool binned = false;
     // ghosts
     int* d_gcnt = nullptr;
@@ -273,7 +276,24 @@ __global__ void k_bin_sort(int mbins, const int* __restrict__ binstart, int* __r
 
 // ---------------------------------------------------------------- neighbour build
 // (neighbor.c:260-352): stencil bins in order, atoms in bin order, rsq <= cutneighsq.
-__global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const double* __restrict__ x,
+__global__ void k_local_flag(int nall, int nlocal, const int* __restrict__ binatoms,
+    int* __restrict__ flag)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nall) flag[m] = binatoms[m] < nlocal;
+}
+
+__global__ void k_local_order(int nall, const int* __restrict__ binatoms,
+    const int* __restrict__ flag, const int* __restrict__ pos, int* __restrict__ order)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nall && flag[m]) order[pos[m]] = binatoms[m];
+}
+
+// One thread per local atom, taken in bin order: the lanes of a wavefront sit in the same or
+// adjacent bins, walk the same stencil columns in step and read the same candidate positions.
+__global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const int* __restrict__ order,
+    const double* __restrict__ x,
     const int* __restrict__ type, int ntypes, Tables tab, int uniform, Grid g,
     const int* __restrict__ binstart, const int* __restrict__ binatoms,
     const int* __restrict__ stencil, int nstencil, int half, int maxneighs,
@@ -283,8 +303,10 @@ __global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const doub
     for (int k = threadIdx.x; k < nstencil; k += blockDim.x)
         s_st[k] = stencil[k];
     __syncthreads();
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlocal) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlocal) return;
+    const int i     = order[t];
+    if (i < 0) return;
     const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
     const int ibin  = coord2bin(g, xt, yt, zt);
     const int ti    = type[i];
@@ -296,18 +318,33 @@ __global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const doub
             continue;
         }
         const int e = binstart[jb + 1];
-        for (int m = binstart[jb]; m < e; m++) {
-            int j = binatoms[m];
-            if (j == i) continue;
-            if (half && j < i) continue;
-            double dx  = xt - x[3 * j];
-            double dy  = yt - x[3 * j + 1];
-            double dz  = zt - x[3 * j + 2];
-            double rsq = dx * dx + dy * dy + dz * dz;
-            double cn  = uniform ? tab.cutn[0] : tab.cutn[ti * ntypes + type[j]];
-            if (rsq <= cn) {
-                if (n < maxneighs) neighbors[(size_t)n * nlocal + i] = j;
-                n++;
+        // four candidates per round: their index and position loads are all in flight at once
+        for (int m = binstart[jb]; m < e; m += 4) {
+            int jv[4];
+            double xv[4], yv[4], zv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                jv[u] = m + u < e ? binatoms[m + u] : i;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                xv[u] = x[3 * jv[u]];
+                yv[u] = x[3 * jv[u] + 1];
+                zv[u] = x[3 * jv[u] + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = jv[u];
+                if (j == i) continue;  // also the padding of the last round
+                if (half && j < i) continue;
+                double dx  = xt - xv[u];
+                double dy  = yt - yv[u];
+                double dz  = zt - zv[u];
+                double rsq = dx * dx + dy * dy + dz * dz;
+                double cn  = uniform ? tab.cutn[0] : tab.cutn[ti * ntypes + type[j]];
+                if (rsq <= cn) {
+                    if (n < maxneighs) neighbors[(size_t)n * nlocal + i] = j;
+                    n++;
+                }
             }
         }
     }
@@ -690,6 +727,9 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
     dfree(S.d_bincount);
     dfree(S.d_binstart);
     dfree(S.d_binatoms);
+    dfree(S.d_lflag);
+    dfree(S.d_lpos);
+    dfree(S.d_order);
     dfree(S.d_gcnt);
     dfree(S.d_goff);
     dfree(S.d_pbc);
@@ -847,6 +887,18 @@ void binatomsGFX950(Atom* atom)
     k_bin_scatter<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, S.d_bin_of, S.d_binstart,
         S.d_bincount, S.d_binatoms);
     k_bin_sort<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, S.d_binstart, S.d_binatoms);
+    // locals in bin order
+    const int nlocal = atom->Nlocal;
+    grow_n(&S.d_lflag, &S.cap_lflag, (size_t)nall + 1, 0);
+    grow_n(&S.d_lpos, &S.cap_lpos, (size_t)nall + 1, 0);
+    grow_n(&S.d_order, &S.cap_order, (size_t)nlocal + 1, 0);
+    if (failed()) return;
+    // an atom outside the grid is in no bin; its slot stays -1 and the build reports the error
+    fill_i32(S.d_order, -1, (size_t)nlocal);
+    k_local_flag<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, nlocal, S.d_binatoms, S.d_lflag);
+    exclusive_scan(S.d_lflag, S.d_lpos, nall, S.d_lpos + nall);
+    k_local_order<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, S.d_binatoms, S.d_lflag,
+        S.d_lpos, S.d_order);
     MDG_LAUNCH_CHECK("binatoms");
     S.binned = true;
 }
@@ -864,7 +916,7 @@ void buildNeighborGFX950(Atom* atom, Neighbor* neighbor)
         fill_i32(S.d_status, 0, 2);
         {
             TimedRegion tr(F_NEIGHBOR);
-            k_build_neighbor<<<blocks_for(nlocal, BLOCK), BLOCK, 0, stream()>>>(nlocal,
+            k_build_neighbor<<<blocks_for(nlocal, BLOCK), BLOCK, 0, stream()>>>(nlocal, S.d_order,
                 atom->d_atom.x, atom->d_atom.type, S.ntypes, S.tab, S.uniform ? 1 : 0, S.g,
                 S.d_binstart, S.d_binatoms, S.d_stencil, S.nstencil, neighbor->half_neigh,
                 neighbor->maxneighs, neighbor->d_neighbor.neighbors,
