@@ -13,6 +13,11 @@ Workloads (seeded fcc lattices, synthetic velocities):
              rho* 0.8442, r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
   cp-lj-64-4x8, cp-lj-64-half: the same with j-clusters of 8 / a half list.
   vl-lj-64   Verlet-list LJ, FP64, full list, same lattice.
+  eam-32     Verlet-list EAM, FP64: 32^3 Cu cells (131,072 atoms), a 3.615 A, synthetic funcfl
+             tables (nr = nrho = 2000, rc 4.95 A), 1600 K, dt 1 fs, reneighbour every 10 steps.
+             eam-64: the same at 64^3 cells (1,048,576 atoms).
+  ka-128     Kob-Andersen 80/20 binary LJ mixture, FP32-mixed cluster pair 4x4: 128^3 cells
+             (8,388,608 atoms), rho 1.2, T 0.8, per-pair cutoff 2.5 sigma_ab.
   hbm-copy   a plain 256 MiB streaming kernel: the scale factor for the HBM byte counters.
 
 `roofline` covers the dominant kernel (the force kernel): algorithmic bytes per launch over its
@@ -201,6 +206,165 @@ def bench_cp(args, mod, N=4, half=0):
         sim.close()
 
 
+def cpu_baseline_eam(nx, steps):
+    cores = host_cores()
+    ref = os.path.join(REPO, "oracle", "_ref", "mdbench-vl-fast")
+    if not os.path.exists(ref):
+        return None
+    import tempfile
+    mod = binding()
+    path = os.path.join(tempfile.gettempdir(), "mdb_bench_cu_%d.eam" % os.getpid())
+    mod.lib().mdbWriteFuncflSynthetic(path.encode())
+    try:
+        v = run_reference(ref, ["-f", "eam", "-e", path, "-nx", str(nx), "-ny", str(nx), "-nz",
+                                str(nx), "-n", str(steps)], cores)
+    finally:
+        os.unlink(path)
+    return {"value": v, "unit": "atom-steps/s", "cores": cores, "kind": "reference",
+            "sample": "%d^3 Cu cells (%d atoms), %d timesteps, MD-Bench EAM -Ofast scalar FP64 "
+                      "(its own EAM setup: a 3.6148 A, 600 K), same synthetic funcfl table"
+                      % (nx, 4 * nx ** 3, steps)}
+
+
+def bench_eam(args, mod, nx=32):
+    import ctypes as C
+    sim = mod.VLSim(nx=nx, ny=nx, nz=nx, force_field=1, lattice_a=3.615, temp=1600.0,
+                    reneigh_every=10, ntimes=args.steps + args.warmup)
+    try:
+        natoms = sim.atom.Natoms
+        sim.run(args.warmup)
+        L = mod.lib()
+        L.mdgfx950Synchronize()
+        L.mdgfx950EnableTiming(1)
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        L.mdgfx950Synchronize()
+        t1 = time.perf_counter()
+        ms = C.c_double()
+        nl = C.c_longlong()
+        L.mdgfx950KernelTime(b"force_eam", C.byref(ms), C.byref(nl))
+        L.mdgfx950EnableTiming(0)
+        st = sim.state()
+        sum_nn = int(st["numneigh"].sum())
+        nlocal = st["nlocal"]
+        elapsed = t1 - t0
+        # Algorithmic bytes of one EAM force call (two passes over the list; spline tables stay in
+        # cache and are not counted).  Density: per atom x_i, count, fp_i write (36 B), per listed
+        # neighbour index + x_j (28 B).  Force: per atom x_i, fp_i, count, f_i write (60 B), per
+        # listed neighbour index + x_j + fp_j (36 B).
+        bytes_launch = nlocal * (36.0 + 60.0) + sum_nn * (28.0 + 36.0)
+        avg_ms = ms.value / max(nl.value, 1)
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        res = {
+            "metric": METRIC, "value": natoms * args.steps / elapsed, "unit": "atom-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "eam-%d: Verlet-list EAM FP64, %d^3 Cu fcc cells (%d atoms), "
+                                   "a 3.615 A, synthetic funcfl nr=nrho=2000 rc 4.95 A, 1600 K, "
+                                   "dt 1 fs, reneigh 10" % (nx, nx, natoms),
+                       "atoms": natoms, "avg_neighbors": sum_nn / nlocal},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_eam_density + k_eam_force (4 lanes/atom)",
+                         "avg_launch_ms": avg_ms, "launches": nl.value,
+                         "bytes_per_launch": bytes_launch},
+        }
+        return res, nx
+    finally:
+        sim.close()
+
+
+KA = dict(ntypes=2, pair_epsilon=[1.0, 1.5, 1.5, 0.5], pair_sigma=[1.0, 0.8, 0.8, 0.88],
+          pair_cutoff=[2.5, 2.0, 2.0, 2.2], rho=1.2, temp=0.8, type1_fraction=0.2, type_seed=4711)
+
+
+def cpu_baseline_ka(nx, steps):
+    """The oracle's FP64 Verlet-list loop (one core) on a sub-box of the mixture: MD-Bench itself
+    has no per-pair sigma table, so it cannot run this mixture."""
+    import ctypes as C
+    so = os.path.join(REPO, "oracle", "liboracle_md.so")
+    if not os.path.exists(so):
+        return None
+    mod = binding()
+    L = C.CDLL(so)
+    x, v, t = mod.create_lattice(nx=nx, ny=nx, nz=nx, rho=1.2, temp=0.8)
+    rng = np.random.default_rng(4711)
+    t = (rng.random(t.size) < 0.2).astype(np.int32)
+    eps = np.array(KA["pair_epsilon"])
+    sig6 = np.array(KA["pair_sigma"]) ** 6
+    cutf = np.array(KA["pair_cutoff"]) ** 2
+    cutn = np.full(4, 2.8 ** 2)
+    lat = (4.0 / 1.2) ** (1.0 / 3.0)
+    f = np.zeros_like(x)
+    ks = np.zeros(steps + 1)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    d = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    L.orc_vl_run.restype = C.c_int
+    t0 = time.perf_counter()
+    rc = L.orc_vl_run(C.c_int(nx), C.c_int(nx), C.c_int(nx), C.c_double(lat), C.c_int(t.size),
+                      d(x), d(v), t.ctypes.data_as(ip), C.c_int(2), d(eps), d(sig6), d(cutf),
+                      d(cutn), C.c_double(2.8), C.c_int(0), C.c_int(0), None, C.c_double(0.005),
+                      C.c_double(0.0025), C.c_int(steps), C.c_int(20), C.c_double(1.0), d(f),
+                      d(ks))
+    t1 = time.perf_counter()
+    if rc != 0:
+        return None
+    return {"value": t.size * steps / (t1 - t0), "unit": "atom-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d^3 cells (%d atoms) of the same mixture, %d timesteps, the FP64 "
+                      "Verlet-list oracle (strict IEEE C, one core)" % (nx, t.size, steps)}
+
+
+def bench_ka(args, mod, nx=128, N=4):
+    import ctypes as C
+    sim = mod.CPSim(nx=nx, ny=nx, nz=nx, ntimes=args.steps + args.warmup, cluster_m=4,
+                    cluster_n=N, precision=0, **KA)
+    try:
+        natoms = 4 * nx ** 3
+        sim.run(args.warmup)
+        L = mod.lib()
+        L.mdgfx950Synchronize()
+        L.mdgfx950EnableTiming(1)
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        L.mdgfx950Synchronize()
+        t1 = time.perf_counter()
+        ms = C.c_double()
+        nl = C.c_longlong()
+        L.mdgfx950KernelTime(b"force_cp", C.byref(ms), C.byref(nl))
+        L.mdgfx950EnableTiming(0)
+        info = sim.info()
+        pairs = int(sim.get("numneigh").astype(np.int64).sum())
+        nci = info["nci"]
+        elapsed = t1 - t0
+        # as for cp-lj-64, plus a 4-byte type per atom: i (16 B per cluster) and j (4 N B per pair)
+        bytes_launch = nci * (8.0 + 96.0 + 48.0 + 16.0) + pairs * (4.0 + 28.0 * N)
+        avg_ms = ms.value / max(nl.value, 1)
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        res = {
+            "metric": METRIC, "value": natoms * args.steps / elapsed, "unit": "atom-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair math, f64 state",
+            "data": "synthetic",
+            "config": {"workload": "ka-%d: Kob-Andersen 80/20 LJ mixture, cluster-pair 4x%d full "
+                                   "list, FP32-mixed, %d^3 fcc cells (%d atoms), rho 1.2, T 0.8, "
+                                   "rc 2.5 sigma_ab, skin 0.3, reneigh 20" % (nx, N, nx, natoms),
+                       "atoms": natoms, "i_clusters": nci,
+                       "j_clusters_per_i_cluster": pairs / nci},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_cp_force_f32 (type-pair table in LDS)",
+                         "avg_launch_ms": avg_ms, "launches": nl.value,
+                         "bytes_per_launch": bytes_launch},
+        }
+        return res, 16
+    finally:
+        sim.close()
+
+
 def bench_copy(args, mod):
     """Plain streaming kernel of known size, the scale for HBM counters: each launch of
     torch.mul(src, 1.0, out=dst) reads and writes 256 MiB (2^26 FP32 values each way)."""
@@ -240,6 +404,9 @@ WORKLOADS = {
     "cp-lj-64-4x8": lambda a, m: bench_cp(a, m, N=8),
     "cp-lj-64-half": lambda a, m: bench_cp(a, m, half=1),
     "vl-lj-64": bench_vl,
+    "eam-32": bench_eam,
+    "eam-64": lambda a, m: bench_eam(a, m, nx=64),
+    "ka-128": bench_ka,
     "hbm-copy": bench_copy,
 }
 DEFAULT = "cp-lj-64"
@@ -262,7 +429,14 @@ def main():
     cb = None
     if not args.no_cpu_baseline and nx is not None:
         try:
-            cb = (cpu_baseline_vl if args.workload.startswith("vl") else cpu_baseline_cp)(nx, 20)
+            if args.workload.startswith("vl"):
+                cb = cpu_baseline_vl(nx, 20)
+            elif args.workload.startswith("eam"):
+                cb = cpu_baseline_eam(min(nx, 32), 20)
+            elif args.workload.startswith("ka"):
+                cb = cpu_baseline_ka(nx, 100)
+            else:
+                cb = cpu_baseline_cp(nx, 20)
         except Exception as exc:  # the GPU figure stands on its own
             print("cpu baseline failed: %s" % exc, file=sys.stderr)
     res["cpu_baseline"] = cb

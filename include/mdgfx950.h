@@ -81,6 +81,12 @@ double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, S
    eam_utils.c:222-284); replaces the global `Eam eam` the reference kernel reads */
 void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double* rhor_spline,
     const double* frho_spline, const double* z2r_spline, int nr_tot, int nrho_tot);
+/* No reference counterpart (the reference computes no EAM energy): while on, every
+   computeForceEamGFX950 call also leaves per-atom energies F(rho_i) + 1/2 sum_j phi(r_ij) on the
+   device, and downloadEamAtomEnergyGFX950 copies the Nlocal values to `out` (returns 0, or -1 with
+   the status set).  Used to label training snapshots for the learned potential. */
+void requestEamAtomEnergyGFX950(int on);
+int downloadEamAtomEnergyGFX950(Atom* atom, double* out);
 /* replace initialIntegrateCUDA / finalIntegrateCUDA (src/verletlist/forceCuda.cu:192-225) */
 void initialIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);
 void finalIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);

@@ -58,6 +58,10 @@ struct State {
     int *d_lflag = nullptr, *d_lpos = nullptr, *d_order = nullptr;
     size_t cap_lflag = 0, cap_lpos = 0, cap_order = 0;
     bool binned = false;
+    // per-atom EAM energies (training labels for the learned potential)
+    double* d_eatom = nullptr;
+    size_t cap_eatom = 0;
+    bool eatom_on = false;
     // ghosts
     int* d_gcnt = nullptr;
     int* d_goff = nullptr;
@@ -454,7 +458,7 @@ template <int LPA, bool STRICT, bool EV>
 __global__ void __launch_bounds__(BLOCK) k_eam_density(int nlocal, const double* __restrict__ x,
     const int* __restrict__ neighbors, const int* __restrict__ numneigh, double cutforcesq,
     const double* __restrict__ rs, const double* __restrict__ fs, int nr, int nrho, double rdr,
-    double rdrho, double* __restrict__ fp, double* __restrict__ ev)
+    double rdrho, double* __restrict__ fp, double* __restrict__ ev, double* __restrict__ eatom)
 {
     __shared__ double s_red[16];
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
@@ -492,7 +496,10 @@ __global__ void __launch_bounds__(BLOCK) k_eam_density(int nlocal, const double*
         p               = p < 1.0 ? p : 1.0;
         const double* c = fs + m * 7;
         fp[i]           = (c[0] * p + c[1]) * p + c[2];
-        if (EV) e = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+        if (EV) {
+            e = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+            if (eatom) eatom[i] = e;
+        }
     }
     if (EV) {
         double be = block_sum(e, s_red);
@@ -512,7 +519,8 @@ template <int LPA, bool STRICT, bool EV>
 __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* __restrict__ x,
     const int* __restrict__ neighbors, const int* __restrict__ numneigh, double cutforcesq,
     const double* __restrict__ rs, const double* __restrict__ zs, int nr, double rdr,
-    const double* __restrict__ fp, double* __restrict__ f, double* __restrict__ ev)
+    const double* __restrict__ fp, double* __restrict__ f, double* __restrict__ ev,
+    double* __restrict__ eatom)
 {
     __shared__ double s_red[16];
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
@@ -568,10 +576,13 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
         fiy = group_sum<LPA>(fiy);
         fiz = group_sum<LPA>(fiz);
     }
+    double ea = 0.0;
+    if (EV && eatom) ea = LPA > 1 ? group_sum<LPA>(e) : e;
     if (act && lane == 0) {
         f[3 * i]     = fix;
         f[3 * i + 1] = fiy;
         f[3 * i + 2] = fiz;
+        if (EV && eatom) eatom[i] += ea;
     }
     if (EV) {
         double be = block_sum(e, s_red);
@@ -727,6 +738,7 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
     dfree(S.d_bincount);
     dfree(S.d_binstart);
     dfree(S.d_binatoms);
+    dfree(S.d_eatom);
     dfree(S.d_lflag);
     dfree(S.d_lpos);
     dfree(S.d_order);
@@ -1082,28 +1094,53 @@ double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, S
     unsigned grid     = blocks_for((long long)nlocal * lpa, BLOCK);
     hipStream_t s     = stream();
     double cutf       = param->cutforce * param->cutforce;
-    if (S.ev_on) fill_f64(S.d_ev, 0.0, 2);
+    double* eatom = nullptr;
+    if (S.eatom_on) {
+        grow_n(&S.d_eatom, &S.cap_eatom, (size_t)nlocal + 1, 0);
+        if (failed()) return 0.0;
+        eatom = S.d_eatom;
+    }
+    const bool ev = S.ev_on || S.eatom_on;
+    if (ev) fill_f64(S.d_ev, 0.0, 2);
     TimedRegion tr(F_FORCE_EAM);
 #define MDG_EAM(L, ST, E)                                                                        \
     do {                                                                                         \
         k_eam_density<L, ST, E><<<grid, BLOCK, 0, s>>>(nlocal, atom->d_atom.x,                   \
             neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
-            S.d_frho, S.nr, S.nrho, S.rdr, S.rdrho, S.d_fp, S.d_ev);                             \
+            S.d_frho, S.nr, S.nrho, S.rdr, S.rdrho, S.d_fp, S.d_ev, eatom);                      \
         launch_ghost_copy(S.d_fp, atom);                                                         \
         k_eam_force<L, ST, E><<<grid, BLOCK, 0, s>>>(nlocal, atom->d_atom.x,                     \
             neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
-            S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, S.d_ev);                              \
+            S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, S.d_ev, eatom);                       \
     } while (0)
     if (strict) {
-        if (S.ev_on) MDG_EAM(1, true, true);
+        if (ev) MDG_EAM(1, true, true);
         else MDG_EAM(1, true, false);
     } else {
-        if (S.ev_on) MDG_EAM(FAST_LPA, false, true);
+        if (ev) MDG_EAM(FAST_LPA, false, true);
         else MDG_EAM(FAST_LPA, false, false);
     }
 #undef MDG_EAM
     MDG_LAUNCH_CHECK("force_eam");
     return wtime() - t0;
+}
+
+void requestEamAtomEnergyGFX950(int on)
+{
+    S.eatom_on = on != 0;
+}
+
+int downloadEamAtomEnergyGFX950(Atom* atom, double* out)
+{
+    if (failed() || !S.ready) return -1;
+    if (!S.eatom_on || !S.d_eatom) {
+        set_error(MDG_ERR_ARG, "per-atom EAM energies were not requested before the force call");
+        return -1;
+    }
+    MDG_HIP(hipMemcpyAsync(out, S.d_eatom, (size_t)atom->Nlocal * 8, hipMemcpyDeviceToHost,
+        stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    return failed() ? -1 : 0;
 }
 
 void initialIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom)

@@ -135,6 +135,9 @@ def lib():
     L.kineticSumGFX950.argtypes = [vp, vp]
     L.kineticSumGFX950.restype = C.c_double
     L.requestEnergyVirialGFX950.argtypes = [C.c_int]
+    L.requestEamAtomEnergyGFX950.argtypes = [C.c_int]
+    L.downloadEamAtomEnergyGFX950.argtypes = [vp, vp]
+    L.downloadEamAtomEnergyGFX950.restype = C.c_int
     L.energyVirialGFX950.argtypes = [_PD, _PD]
     L.copyDataToGFX950Device.argtypes = [vp, vp]
     L.copyDataFromGFX950Device.argtypes = [vp, vp]
@@ -292,6 +295,19 @@ class VLSim:
         e, w = C.c_double(), C.c_double()
         lib().energyVirialGFX950(C.byref(e), C.byref(w))
         return e.value, w.value
+
+    def eam_atom_energy(self):
+        """Forces and per-atom energies F(rho_i) + 1/2 sum phi of the current positions."""
+        L = lib()
+        L.requestEamAtomEnergyGFX950(1)
+        try:
+            self.compute_force()
+            out = np.zeros(self.atom.Nlocal)
+            rc = L.downloadEamAtomEnergyGFX950(self.atom_p, _ptr(out))
+            assert rc == 0, status()
+        finally:
+            L.requestEamAtomEnergyGFX950(0)
+        return out
 
 
 class CPSim:

@@ -166,10 +166,12 @@ void orc_eam_free(orc_funcfl* file, orc_eam* eam)
     }
 }
 
-/* computeForceEam: src/verletlist/force_eam.c:19-228 (CSR list, one table). */
-void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* x,
+/* computeForceEam: src/verletlist/force_eam.c:19-228 (CSR list, one table).  eatom (may be NULL)
+   receives the per-atom energy F(rho_i) + 1/2 sum_j phi(r_ij) (new: the reference computes no
+   energy). */
+static void force_eam(const orc_eam* eam, int nlocal, int nghost, const double* x,
     const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
-    double* fp, double* f, double* energy)
+    double* fp, double* f, double* energy, double* eatom)
 {
     const double* rs = eam->rhor_spline;
     const double* fs = eam->frho_spline;
@@ -206,7 +208,9 @@ void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* 
         fp[i] = (fs[m * 7 + 0] * p + fs[m * 7 + 1]) * p + fs[m * 7 + 2];
         /* Embedding energy F(rho_i) from the value coefficients (new: the reference computes no
            energy). */
-        e += ((fs[m * 7 + 3] * p + fs[m * 7 + 4]) * p + fs[m * 7 + 5]) * p + fs[m * 7 + 6];
+        const double emb = ((fs[m * 7 + 3] * p + fs[m * 7 + 4]) * p + fs[m * 7 + 5]) * p + fs[m * 7 + 6];
+        e += emb;
+        if (eatom) eatom[i] = emb;
         off += nn;
     }
     /* ghost fp copy (:115-118) */
@@ -243,6 +247,7 @@ void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* 
                 fiy += dely * fpair;
                 fiz += delz * fpair;
                 e += 0.5 * phi;
+                if (eatom) eatom[i] += 0.5 * phi;
             }
         }
         f[3 * i]     = fix;
@@ -251,4 +256,20 @@ void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* 
         off += nn;
     }
     if (energy) *energy = e;
+}
+
+void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* x,
+    const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
+    double* fp, double* f, double* energy)
+{
+    force_eam(eam, nlocal, nghost, x, border_map, cutforcesq, numneigh, neighbors, fp, f, energy,
+        NULL);
+}
+
+void orc_vl_force_eam_atom(const orc_eam* eam, int nlocal, int nghost, const double* x,
+    const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
+    double* fp, double* f, double* eatom)
+{
+    force_eam(eam, nlocal, nghost, x, border_map, cutforcesq, numneigh, neighbors, fp, f, NULL,
+        eatom);
 }

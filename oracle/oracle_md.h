@@ -87,6 +87,11 @@ void orc_eam_free(orc_funcfl* file, orc_eam* eam);
 void orc_vl_force_eam(const orc_eam* eam, int nlocal, int nghost, const double* x,
     const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
     double* fp, double* f, double* energy);
+/* The same forces; eatom[i] = F(rho_i) + 1/2 sum_j phi(r_ij) per local atom (their sum is the
+   energy above; pinned by finite differences of the sum). */
+void orc_vl_force_eam_atom(const orc_eam* eam, int nlocal, int nghost, const double* x,
+    const int* border_map, double cutforcesq, const int* numneigh, const int* neighbors,
+    double* fp, double* f, double* eatom);
 
 /* Velocity Verlet (src/verletlist/integrate.c:21-42). */
 void orc_initial_integrate(int n, double* x, double* v, const double* f, double dt,

@@ -66,6 +66,8 @@ def lib():
     L.orc_eam_free.argtypes = [C.POINTER(Funcfl), C.POINTER(Eam)]
     L.orc_vl_force_eam.argtypes = [C.POINTER(Eam), C.c_int, C.c_int, _dp, _ip, C.c_double, _ip,
                                    _ip, _dp, _dp, C.POINTER(C.c_double)]
+    L.orc_vl_force_eam_atom.argtypes = [C.POINTER(Eam), C.c_int, C.c_int, _dp, _ip, C.c_double,
+                                        _ip, _ip, _dp, _dp, _dp]
     L.orc_initial_integrate.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double]
     L.orc_final_integrate.argtypes = [C.c_int, _dp, _dp, C.c_double]
     L.orc_kinetic_sum.argtypes = [C.c_int, _dp, C.c_double]
@@ -166,6 +168,18 @@ def force_eam(tab, nlocal, x, bmap, cutforcesq, numneigh, nb):
                            np.ascontiguousarray(x, np.float64), bmap, cutforcesq, numneigh, nb,
                            fp, f, C.byref(e))
     return f, fp, e.value
+
+
+def force_eam_atom(tab, nlocal, x, bmap, cutforcesq, numneigh, nb):
+    """EAM forces and per-atom energies F(rho_i) + 1/2 sum_j phi(r_ij)."""
+    nall = x.size // 3
+    fp = np.zeros(nall, np.float64)
+    f = np.zeros(3 * nlocal, np.float64)
+    ea = np.zeros(nlocal, np.float64)
+    lib().orc_vl_force_eam_atom(C.byref(tab.eam), nlocal, nall - nlocal,
+                                np.ascontiguousarray(x, np.float64), bmap, cutforcesq, numneigh,
+                                nb, fp, f, ea)
+    return f, ea
 
 
 def kinetic_sum(v, mass):

@@ -158,3 +158,36 @@ def test_default_testcase_regression(strict):
         assert abs(t - 7.961535e-01) / 7.961535e-01 <= (1e-6 if strict else 1e-4)
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("strict", [1, 0])
+def test_eam_atom_energy_matches_oracle(strict):
+    """Per-atom EAM energies (the learned potential's training labels) on a thermalised 6^3 Cu
+    box against the oracle on the downloaded state; their sum is the total energy."""
+    import tempfile
+    from golden.eam_tables import write_funcfl_synthetic
+    P = product()
+    sim = P.VLSim(nx=6, ny=6, nz=6, force_field=1, lattice_a=3.615, temp=1600.0, ntimes=40,
+                  reneigh_every=10, strict=strict)
+    try:
+        sim.run(40)
+        ea = sim.eam_atom_energy()
+        st = sim.state()
+        nl = sim.atom.Nlocal
+        path = os.path.join(tempfile.gettempdir(), "mdb_cu_synth_gpu_%d.eam" % os.getpid())
+        write_funcfl_synthetic(path)
+        tab = ob.EamTables(path)
+        cutf = sim.param.cutforce ** 2
+        f, ear = ob.force_eam_atom(tab, nl, st["x"], st["border_map"], cutf, st["numneigh"],
+                                   st["neighbors"])
+        assert _relerr(st["f"], f) < FORCE_RTOL
+        # the pair half-sum is added to F(rho_i) after the neighbour loop (the oracle adds term
+        # by term), so the two differ by rounding only
+        assert np.max(np.abs(ea - ear)) <= 1e-12 * np.max(np.abs(ear))
+        P.lib().requestEnergyVirialGFX950(1)
+        sim.compute_force()
+        e, _ = sim.energy_virial()
+        P.lib().requestEnergyVirialGFX950(0)
+        assert abs(e - ear.sum()) <= 1e-11 * abs(e)
+    finally:
+        sim.close()

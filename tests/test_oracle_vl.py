@@ -116,3 +116,42 @@ def test_eam_tables_and_force(case):
     f, fp, e = ob.force_eam(tab, nlocal, d["s0_x"], d["s0_border_map"], cutf, d["s0_numneigh"],
                             d["s0_neighbors"])
     np.testing.assert_array_equal(f, d["s0_f"])
+
+
+@pytest.mark.parametrize("case", EAM_CASES)
+def test_eam_atom_energy_sums_to_total_and_is_the_force_potential(case):
+    """Per-atom EAM energies F(rho_i) + 1/2 sum_j phi(r_ij): they add up to the total energy, and
+    central differences of their sum reproduce the forces (moving an atom moves its ghost images
+    with it; the neighbour list stays valid for the small step)."""
+    d = load_golden(case)
+    args = [str(a) for a in d["args"]]
+    tab = ob.EamTables(_eam_path(args[7]))
+    nlocal = meta(d, "s0_nlocal")
+    cutf = d["cutforcesq"][0]
+    x0 = np.array(d["s0_x"], np.float64)
+    bmap = d["s0_border_map"]
+    nn, nb = d["s0_numneigh"], d["s0_neighbors"]
+    f, ea = ob.force_eam_atom(tab, nlocal, x0, bmap, cutf, nn, nb)
+    f2, _, e = ob.force_eam(tab, nlocal, x0, bmap, cutf, nn, nb)
+    np.testing.assert_array_equal(f, f2)
+    assert abs(ea.sum() - e) <= 1e-12 * abs(e)
+    # thermal displacements so that forces are not zero by symmetry
+    rng = np.random.default_rng(5)
+    x = x0.copy()
+    dx = rng.normal(0.0, 0.05, size=(nlocal, 3))
+    x[:3 * nlocal] += dx.reshape(-1)
+    img = x0[3 * nlocal:].reshape(-1, 3) - x0[:3 * nlocal].reshape(-1, 3)[bmap]
+    x[3 * nlocal:] = (x[:3 * nlocal].reshape(-1, 3)[bmap] + img).reshape(-1)
+    f, _ = ob.force_eam_atom(tab, nlocal, x, bmap, cutf, nn, nb)
+    h = 1e-5
+    for k in [0, 7, nlocal // 2, nlocal - 1]:
+        for c in range(3):
+            es = []
+            for sgn in (1, -1):
+                xs = x.copy()
+                xs[3 * k + c] += sgn * h
+                ghosts = np.nonzero(bmap == k)[0]
+                xs[3 * (nlocal + ghosts) + c] += sgn * h
+                es.append(ob.force_eam_atom(tab, nlocal, xs, bmap, cutf, nn, nb)[1].sum())
+            fd = -(es[0] - es[1]) / (2 * h)
+            assert abs(fd - f[3 * k + c]) <= 1e-6 * max(1.0, abs(f[3 * k + c])), (k, c, fd)
