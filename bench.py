@@ -72,7 +72,7 @@ def run_reference(binary, args, cores):
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_SCHEDULE="static",
                OMP_PROC_BIND="close")
     out = subprocess.run([binary] + args, env=env, capture_output=True, text=True, timeout=300,
-                         cwd="/tmp")
+                         cwd="/tmp", errors="replace")
     m = re.search(r"([0-9.]+) atom updates/us", out.stdout)
     if out.returncode != 0 or not m:
         raise RuntimeError("reference run failed: %s" % out.stdout[-500:])

@@ -32,6 +32,12 @@ Stats* mdbVLStats(mdb_vl* s);
 void mdbThermo(const mdb_vl* s, double ksum, double* temp, double* press);
 /* Atoms after setup, before the device upload: x (3N), v (3N, adjusted), type (N). */
 int mdbVLInitialState(const mdb_vl* s, double* x, double* v, int* type);
+/* EAM runs only: forces from the learned surrogate (model set with surSetModelGFX950), judged
+   against plain EAM every judge_every steps; above tol (RMS per force component, eV/A) the run
+   falls back to plain EAM for good.  on = 0 returns to plain EAM. */
+int mdbVLUseSurrogate(mdb_vl* s, int on, int judge_every, double tol);
+/* out[5]: judges run, last RMS, largest RMS, step of the fallback (-1: none), fallback flag */
+void mdbVLSurrogateStats(const mdb_vl* s, double* out);
 
 #ifdef __cplusplus
 }

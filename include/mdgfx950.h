@@ -87,6 +87,15 @@ void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double
    the status set).  Used to label training snapshots for the learned potential. */
 void requestEamAtomEnergyGFX950(int on);
 int downloadEamAtomEnergyGFX950(Atom* atom, double* out);
+/* No reference counterpart: the learned EAM surrogate (mdgfx950_surrogate.h) as a computeForce
+   (signature of src/verletlist/force.h:22-24): descriptors, network and forces of all local atoms
+   into atom->d_atom.fx, with the model set by surSetModelGFX950.  Returns seconds spent. */
+double computeForceSurrogateGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
+/* The judge: with atom->d_atom.fx holding surrogate forces, computes the plain EAM forces of the
+   same positions and returns the RMS difference per force component (eV/A).  At or below tol the
+   surrogate forces are put back; above it the EAM forces stay and *fallback is set to 1. */
+double surrogateJudgeGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats,
+    double tol, int* fallback);
 /* replace initialIntegrateCUDA / finalIntegrateCUDA (src/verletlist/forceCuda.cu:192-225) */
 void initialIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);
 void finalIntegrateGFX950(bool reneigh, Parameter* param, Atom* atom);

@@ -322,33 +322,18 @@ __global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const int*
             continue;
         }
         const int e = binstart[jb + 1];
-        // four candidates per round: their index and position loads are all in flight at once
-        for (int m = binstart[jb]; m < e; m += 4) {
-            int jv[4];
-            double xv[4], yv[4], zv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                jv[u] = m + u < e ? binatoms[m + u] : i;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                xv[u] = x[3 * jv[u]];
-                yv[u] = x[3 * jv[u] + 1];
-                zv[u] = x[3 * jv[u] + 2];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int j = jv[u];
-                if (j == i) continue;  // also the padding of the last round
-                if (half && j < i) continue;
-                double dx  = xt - xv[u];
-                double dy  = yt - yv[u];
-                double dz  = zt - zv[u];
-                double rsq = dx * dx + dy * dy + dz * dz;
-                double cn  = uniform ? tab.cutn[0] : tab.cutn[ti * ntypes + type[j]];
-                if (rsq <= cn) {
-                    if (n < maxneighs) neighbors[(size_t)n * nlocal + i] = j;
-                    n++;
-                }
+        for (int m = binstart[jb]; m < e; m++) {
+            int j = binatoms[m];
+            if (j == i) continue;
+            if (half && j < i) continue;
+            double dx  = xt - x[3 * j];
+            double dy  = yt - x[3 * j + 1];
+            double dz  = zt - x[3 * j + 2];
+            double rsq = dx * dx + dy * dy + dz * dz;
+            double cn  = uniform ? tab.cutn[0] : tab.cutn[ti * ntypes + type[j]];
+            if (rsq <= cn) {
+                if (n < maxneighs) neighbors[(size_t)n * nlocal + i] = j;
+                n++;
             }
         }
     }

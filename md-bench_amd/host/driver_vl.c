@@ -22,6 +22,10 @@ struct mdb_vl {
     double* v0;
     int* t0;
     int device_ready;
+    /* learned EAM surrogate (force_field FF_EAM only): judged by plain EAM every sur_every steps */
+    int sur_on, sur_every, sur_fallback, sur_judges;
+    double sur_tol, sur_last, sur_max;
+    long long steps_done, sur_fallback_step;
 };
 
 Parameter* mdbVLParam(mdb_vl* s) { return &s->param; }
@@ -55,8 +59,11 @@ void mdb_free_atom_host(Atom* a)
 
 static double compute_force(mdb_vl* s)
 {
-    if (s->param.force_field == FF_EAM)
+    if (s->param.force_field == FF_EAM) {
+        if (s->sur_on && !s->sur_fallback)
+            return computeForceSurrogateGFX950(&s->param, &s->atom, &s->neighbor, &s->stats);
         return computeForceEamGFX950(&s->param, &s->atom, &s->neighbor, &s->stats);
+    }
     return computeForceLJGFX950(&s->param, &s->atom, &s->neighbor, &s->stats);
 }
 
@@ -224,11 +231,48 @@ int mdbVLRun(mdb_vl* s, int nsteps, double* ksum)
             updatePbcGFX950(a, p, false);
         }
         compute_force(s);
+        s->steps_done++;
+        if (s->sur_on && !s->sur_fallback && s->sur_every > 0 &&
+            s->steps_done % s->sur_every == 0) {
+            int fb   = 0;
+            double r = surrogateJudgeGFX950(p, a, nb, &s->stats, s->sur_tol, &fb);
+            s->sur_judges++;
+            s->sur_last = r;
+            if (r > s->sur_max) s->sur_max = r;
+            if (fb) {
+                s->sur_fallback      = 1;
+                s->sur_fallback_step = s->steps_done;
+            }
+        }
         finalIntegrateGFX950(reneigh, p, a);
         if (ksum) ksum[n] = kineticSumGFX950(p, a);
         if (mdgfx950Status()) return mdgfx950Status();
     }
     return mdgfx950Status();
+}
+
+int mdbVLUseSurrogate(mdb_vl* s, int on, int judge_every, double tol)
+{
+    if (s->param.force_field != FF_EAM) return MDG_ERR_ARG;
+    s->sur_on            = on != 0;
+    s->sur_every         = judge_every;
+    s->sur_tol           = tol;
+    s->sur_fallback      = 0;
+    s->sur_judges        = 0;
+    s->sur_last          = 0.0;
+    s->sur_max           = 0.0;
+    s->sur_fallback_step = -1;
+    if (s->sur_on) compute_force(s); /* forces of the current positions from the surrogate */
+    return mdgfx950Status();
+}
+
+void mdbVLSurrogateStats(const mdb_vl* s, double* out)
+{
+    out[0] = s->sur_judges;
+    out[1] = s->sur_last;
+    out[2] = s->sur_max;
+    out[3] = (double)s->sur_fallback_step;
+    out[4] = s->sur_fallback;
 }
 
 void mdbVLDestroy(mdb_vl* s)

@@ -32,7 +32,7 @@ A_MU0, A_DMU = 2.5, 0.6        # channel centres 2.5, 3.1, 3.7, 4.3 A
 A_ETA = 1.0 / (2.0 * 0.35 ** 2)
 ND = NR + 2 * NA               # 32 descriptors
 H = 64                         # hidden width
-NCOEF = NR + NA * (3 + 6 + 1)  # 64 force coefficients per atom
+NCOEF = 64                     # force coefficients per atom: 24 + 4 x (3 + 6), padded
 SYM = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]  # storage order of a symmetric 3x3
 
 
@@ -73,9 +73,10 @@ def _pairs(x, nlocal, numneigh, neighbors):
 
 
 def descriptors(x, nlocal, numneigh, neighbors):
-    """D (nlocal, 32): 24 radial G_n, then q1_c = |A_c|^2 and q2_c = 1.5 |T_c|_F^2 - 0.5 tr(T_c)^2
-    for the four channels, with A_c = sum_j w_c u_j and T_c = sum_j w_c u_j u_j^T.  Also returns
-    the moments A (nlocal, 4, 3) and T (nlocal, 4, 3, 3) the force pass needs.
+    """D (nlocal, 32): 24 radial G_n, then q1_c = |A_c|^2 and q2_c = 1.5 |T~_c|_F^2 for the four
+    channels, with A_c = sum_j w_c u_j, T_c = sum_j w_c u_j u_j^T and T~_c = T_c - tr(T_c) I / 3
+    (1.5 |T~|^2 = 1.5 |T|^2 - 0.5 tr(T)^2 without the cancellation).  Also returns the moments A
+    (nlocal, 4, 3) and T~ (nlocal, 4, 3, 3) the force pass needs.
     q1_c = sum_jk w_j w_k cos(theta_jk), q2_c = sum_jk w_j w_k P2(cos(theta_jk))."""
     i, j, d, r = _pairs(x, nlocal, numneigh, neighbors)
     u = d / r[:, None]
@@ -89,7 +90,8 @@ def descriptors(x, nlocal, numneigh, neighbors):
     np.add.at(T, i, W[:, :, None, None] * (u[:, :, None] * u[:, None, :])[:, None])
     D[:, NR:NR + NA] = (A * A).sum(-1)
     tr = np.trace(T, axis1=2, axis2=3)
-    D[:, NR + NA:] = 1.5 * (T * T).sum((-1, -2)) - 0.5 * tr ** 2
+    T = T - tr[..., None, None] * np.eye(3) / 3.0
+    D[:, NR + NA:] = 1.5 * (T * T).sum((-1, -2))
     return D, A, T
 
 
@@ -191,14 +193,14 @@ def mlp_input_grad(p, cache, bf16_inputs=False):
 # ---------------------------------------------------------------- forces
 def coefficients(g, A, T):
     """Per-atom force coefficients (nlocal, 64): [24 radial g_n][4 x 3: 2 g_q1 A_c]
-    [4 x 6: 3 g_q2 T_c, symmetric storage xx yy zz xy xz yz][4: g_q2 tr T_c]."""
+    [4 x 6: 3 g_q2 T~_c, symmetric storage xx yy zz xy xz yz][4 unused, zero].  T is the traceless
+    moment from descriptors(); g . dq2/dx_j = 3 g T~ : dT/dx_j."""
     n = g.shape[0]
     c = np.zeros((n, NCOEF))
     c[:, :NR] = g[:, :NR]
     c[:, NR:NR + 3 * NA] = (2.0 * g[:, NR:NR + NA, None] * A).reshape(n, -1)
     B = 3.0 * g[:, NR + NA:, None, None] * T
     c[:, NR + 3 * NA:NR + 9 * NA] = np.stack([B[:, :, a, b] for a, b in SYM], -1).reshape(n, -1)
-    c[:, NR + 9 * NA:] = g[:, NR + NA:] * np.trace(T, axis1=2, axis2=3)
     return c
 
 
@@ -277,9 +279,7 @@ def descriptor_jvp(x, nlocal, numneigh, neighbors, A, T, e_pair):
     Tu = np.einsum("pcab,pb->pca", Ti, u)
     uTu = np.einsum("pca,pa->pc", Tu, u)
     eTu = np.einsum("pca,pa->pc", Tu, e_pair)
-    tr = np.trace(Ti, axis1=2, axis2=3)
-    q2 = 3.0 * (dW * uTu * ue[:, None] + 2.0 * wr * (eTu - uTu * ue[:, None])) \
-        - tr * dW * ue[:, None]
+    q2 = 3.0 * (dW * uTu * ue[:, None] + 2.0 * wr * (eTu - uTu * ue[:, None]))
     np.add.at(out[:, NR + NA:], i, q2)
     return out
 

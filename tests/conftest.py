@@ -41,3 +41,17 @@ def product():
     sys.modules["mdbench_gfx950"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def surrogate_module():
+    """The surrogate's host module (md-bench_amd/eam_surrogate.py)."""
+    import importlib.util
+    product()
+    if "eam_surrogate" in sys.modules:
+        return sys.modules["eam_surrogate"]
+    path = os.path.join(REPO, "md-bench_amd", "eam_surrogate.py")
+    spec = importlib.util.spec_from_file_location("eam_surrogate", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["eam_surrogate"] = mod
+    spec.loader.exec_module(mod)
+    return mod

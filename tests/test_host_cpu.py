@@ -13,7 +13,9 @@ from conftest import REPO, load_golden, meta, product
 
 def _declared_functions():
     names = []
-    for fn in ["mdgfx950.h", "mdbench_driver.h"]:
+    for fn in sorted(os.listdir(os.path.join(REPO, "include"))):
+        if not fn.endswith(".h"):
+            continue
         src = open(os.path.join(REPO, "include", fn)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", src, flags=re.M):
