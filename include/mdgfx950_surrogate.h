@@ -53,15 +53,19 @@ int surSetParamsDeviceGFX950(const float* theta_dev);
 int surGetParamsGFX950(float* theta_host);
 
 /* Descriptors of atoms i0 .. i0+n-1: dn (device, n x 32) normalised with the model's
-   statistics (pass identity statistics to get raw values), mom (device, n x 36). */
-int surDescriptorsGFX950(const MdgSurSystem* sys, int i0, int n, float* dn, float* mom);
+   statistics (pass identity statistics to get raw values), mom (device, n x 36: per angular
+   channel A (3) and the traceless T (6)), and the list of pairs inside the descriptor cutoff
+   (device: pairs[k * n + a], k < npairs[a] <= cap; cap >= the longest Verlet list). */
+int surDescriptorsGFX950(const MdgSurSystem* sys, int i0, int n, float* dn, float* mom, int cap,
+    int* pairs, int* npairs);
 /* MLP forward and input gradient (bfloat16 MFMA): per-atom energies e (device, n, may be NULL)
-   and force coefficients coef (device, n x 64) from dn and mom. */
-int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, float* coef);
+   and force coefficients coef (device, n x 64 IEEE halves) from dn and mom. */
+int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, void* coef);
 /* Forces of atoms i0 .. i0+n-1 into f (device, n x 3 doubles) from coef (rows i - i0; a ghost
    neighbour uses its owner's row, which must lie in the same range: pairs whose row falls
-   outside it are skipped). */
-int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const float* coef, double* f);
+   outside it are skipped) over the pair list of surDescriptorsGFX950. */
+int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, const int* npairs,
+    const void* coef, double* f);
 
 /* The drop-in force call computeForceSurrogateGFX950 and the judge surrogateJudgeGFX950 work on
    the Verlet-list structs and are declared in mdgfx950.h. */
@@ -72,8 +76,8 @@ int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const float* coef, d
 int surLossTermsGFX950(int n, const float* e, const double* f, const double* e_ref,
     const double* f_ref, float wE, int n_norm, float* cE, float* dF, double* sums);
 /* U = scale * 2 sum_j J_ij^T (dF_i - dF_owner(j)) (device, n x 32): dL/dg for the force term. */
-int surAdjointGFX950(const MdgSurSystem* sys, int i0, int n, const float* mom, const float* dF,
-    float scale, float* U);
+int surAdjointGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, const int* npairs,
+    const float* mom, const float* dF, float scale, float* U);
 /* grad (device, MDG_SUR_NPARAM) += d/dtheta sum_i [cE_i E_i + U_i . dE_i/dD_i] over n atoms
    (FP32, exact network arithmetic). */
 int surWeightGradGFX950(int n, const float* dn, const float* U, const float* cE, float* grad);

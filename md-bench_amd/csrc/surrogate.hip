@@ -16,6 +16,7 @@
 // Training adds the force adjoint (k_sur_adjoint), the exact FP32 weight gradient of the loss
 // (k_sur_wgrad: forward mode along the adjoint, then reverse mode, block partials reduced in a
 // fixed order) and Adam (k_sur_adam).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -56,8 +57,10 @@ struct SurModel {
     float e0 = 0.f, s_e = 1.f;
     bool ready = false;
     // per-atom work arrays of the drop-in force call
-    float *dn = nullptr, *mom = nullptr, *coef = nullptr;
-    size_t cap_dn = 0, cap_mom = 0, cap_coef = 0;
+    float *dn = nullptr, *mom = nullptr;
+    __half* coef = nullptr;
+    int *pairs = nullptr, *npairs = nullptr;
+    size_t cap_dn = 0, cap_mom = 0, cap_coef = 0, cap_pairs = 0, cap_npairs = 0;
     double* fsave = nullptr;
     size_t cap_fsave = 0;
     // scratch of the weight gradient (block partials) and of reductions
@@ -70,27 +73,33 @@ struct SurModel {
 SurModel M;
 
 // ---------------------------------------------------------------- pair geometry
-// Gaussians exp(-eta (r - mu_n)^2) by a multiplicative recurrence from the middle centre: the
-// ratio of neighbours is exp(+-2 eta D (r - mu_n) - eta D^2) and changes by exp(-2 eta D^2) per
-// step, so three exponentials give all 24 values, and no intermediate over- or underflows
-// spuriously (values shrink monotonically away from the middle when they are small).
-__device__ inline void gaussians(float r, float* g)
+// The LPA = 4 lanes of an atom walk its list together, one pair at a time, and split the
+// features: lane q owns radial descriptors 6 q .. 6 q + 5 and angular channel q.  Per-atom data
+// is laid out in matching lane blocks, so the four lanes read one contiguous row:
+//   coefficients (64 halves): block q = [c_{6q} .. c_{6q+5}, a_q (3), M_q (6: xx yy zz xy xz yz), 0]
+//   moments      (36 floats): block q = [A_q (3), T~_q (6)]
+constexpr int RPL = NR / LPA;  // radial descriptors per lane (6)
+
+// exp(-eta (r - mu_n)^2) for the lane's six centres by a multiplicative recurrence from the
+// lane's third centre: the ratio of neighbours is exp(+-2 eta D (r - mu_n) - eta D^2) and changes
+// by exp(-2 eta D^2) per step.  Three exponentials give six values; a value can underflow only
+// when it is below 1e-20 and negligible.
+__device__ inline void gaussians6(float r, int q, float* g)
 {
-    const float t  = r - (R_MU0 + RMID * R_DMU);
-    const float c  = __expf(-2.0f * R_ETA * R_DMU * R_DMU);
-    float up       = __expf(2.0f * R_ETA * R_DMU * t - R_ETA * R_DMU * R_DMU);
-    float dn       = __expf(-2.0f * R_ETA * R_DMU * t - R_ETA * R_DMU * R_DMU);
-    g[RMID]        = __expf(-R_ETA * t * t);
-#pragma unroll
-    for (int n = RMID + 1; n < NR; n++) {
-        g[n] = g[n - 1] * up;
-        up *= c;
-    }
-#pragma unroll
-    for (int n = RMID - 1; n >= 0; n--) {
-        g[n] = g[n + 1] * dn;
-        dn *= c;
-    }
+    const float t   = r - (R_MU0 + (RPL * q + 3) * R_DMU);
+    const float e2  = R_ETA * R_DMU * R_DMU;
+    const float c   = __expf(-2.0f * e2);
+    float up        = __expf(2.0f * R_ETA * R_DMU * t - e2);
+    float dn        = __expf(-2.0f * R_ETA * R_DMU * t - e2);
+    g[3]            = __expf(-R_ETA * t * t);
+    g[4]            = g[3] * up;
+    up *= c;
+    g[5]            = g[4] * up;
+    g[2]            = g[3] * dn;
+    dn *= c;
+    g[1]            = g[2] * dn;
+    dn *= c;
+    g[0]            = g[1] * dn;
 }
 
 __device__ inline void cutoff_fn(float r2, float r, float* fc, float* dfc)
@@ -120,17 +129,64 @@ __device__ inline float gsum(float v)
     return v;
 }
 
+// Geometry of one pair, evaluated by each of the atom's lanes.
+struct Pair {
+    float r, ir, ux, uy, uz, fc, dfc;
+};
+
+__device__ inline bool pair_geom(const double* __restrict__ x, double xi, double yi, double zi,
+    int j, Pair* p)
+{
+    const float dx = (float)(x[3 * j] - xi);
+    const float dy = (float)(x[3 * j + 1] - yi);
+    const float dz = (float)(x[3 * j + 2] - zi);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < RC2)) return false;
+    p->r  = __builtin_sqrtf(r2);
+    p->ir = rcp_refined(p->r);
+    p->ux = dx * p->ir;
+    p->uy = dy * p->ir;
+    p->uz = dz * p->ir;
+    cutoff_fn(r2, p->r, &p->fc, &p->dfc);
+    return true;
+}
+
 // ---------------------------------------------------------------- descriptors
+// Here the atom's lanes split the pairs (lane q takes list entries q, q + 4, ...) and each
+// accumulates all 60 moments; the lanes then add theirs.  The pairs inside rc are also written,
+// in list order, to a compact list (SoA, cl[k * n + a]) for the force and adjoint passes.
+__device__ inline void gaussians24(float r, float* g)
+{
+    const float t  = r - (R_MU0 + RMID * R_DMU);
+    const float e2 = R_ETA * R_DMU * R_DMU;
+    const float c  = __expf(-2.0f * e2);
+    float up       = __expf(2.0f * R_ETA * R_DMU * t - e2);
+    float dn       = __expf(-2.0f * R_ETA * R_DMU * t - e2);
+    g[RMID]        = __expf(-R_ETA * t * t);
+#pragma unroll
+    for (int m = RMID + 1; m < NR; m++) {
+        g[m] = g[m - 1] * up;
+        up *= c;
+    }
+#pragma unroll
+    for (int m = RMID - 1; m >= 0; m--) {
+        g[m] = g[m + 1] * dn;
+        dn *= c;
+    }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_sur_desc(int nlocal, int i0, int n,
     const double* __restrict__ x, const int* __restrict__ neighbors,
     const int* __restrict__ numneigh, const float* __restrict__ norm, float* __restrict__ dn_out,
-    float* __restrict__ mom_out)
+    float* __restrict__ mom_out, int cap, int* __restrict__ cl, int* __restrict__ ncl)
 {
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
     const int a    = gid / LPA;
-    const int lane = gid % LPA;
+    const int q    = gid % LPA;
     const bool act = a < n;
     const int i    = i0 + (act ? a : 0);
+    const int lane = threadIdx.x & 63;
+    const int gbase = lane & ~(LPA - 1);
     float G[NR], A[NA][3], T[NA][6];
 #pragma unroll
     for (int k = 0; k < NR; k++)
@@ -142,39 +198,48 @@ __global__ void __launch_bounds__(BLOCK) k_sur_desc(int nlocal, int i0, int n,
         for (int k = 0; k < 6; k++)
             T[c][k] = 0.f;
     }
-    if (act) {
-        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-        const int nn = numneigh[i];
-        for (int k = lane; k < nn; k += LPA) {
-            const int j    = neighbors[(size_t)k * nlocal + i];
-            const float dx = (float)(x[3 * j] - xi);
-            const float dy = (float)(x[3 * j + 1] - yi);
-            const float dz = (float)(x[3 * j + 2] - zi);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < RC2)) continue;
-            const float r    = __builtin_sqrtf(r2);
-            const float ir   = rcp_refined(r);
-            const float ux = dx * ir, uy = dy * ir, uz = dz * ir;
-            float fc, dfc;
-            cutoff_fn(r2, r, &fc, &dfc);
+    int nc = 0;  // compact-list length (the same in the atom's four lanes)
+    const int nn = act ? numneigh[i] : 0;
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    // the wavefront walks in step so that the ballot below sees every lane
+    int nmax = nn;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    for (int k0 = 0; k0 < nmax; k0 += LPA) {
+        const int k = k0 + q;
+        Pair p;
+        bool in = false;
+        int j   = 0;
+        if (k < nn) {
+            j  = neighbors[(size_t)k * nlocal + i];
+            in = pair_geom(x, xi, yi, zi, j, &p);
+        }
+        const unsigned long long bal = __ballot(in);
+        const unsigned grp = (unsigned)(bal >> gbase) & ((1u << LPA) - 1u);
+        if (in) {
+            const int slot = nc + __popc(grp & ((1u << q) - 1u));
+            if (slot < cap) cl[(size_t)slot * n + a] = j;
             float g[NR];
-            gaussians(r, g);
+            gaussians24(p.r, g);
 #pragma unroll
             for (int m = 0; m < NR; m++)
-                G[m] = fmaf(g[m], fc, G[m]);
-            const float uu[6] = { ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz };
+                G[m] = fmaf(g[m], p.fc, G[m]);
+            const float uu[6] = { p.ux * p.ux, p.uy * p.uy, p.uz * p.uz, p.ux * p.uy,
+                p.ux * p.uz, p.uy * p.uz };
 #pragma unroll
             for (int c = 0; c < NA; c++) {
-                const float dr = r - (A_MU0 + c * A_DMU);
-                const float w  = __expf(-A_ETA * dr * dr) * fc;
-                A[c][0] = fmaf(w, ux, A[c][0]);
-                A[c][1] = fmaf(w, uy, A[c][1]);
-                A[c][2] = fmaf(w, uz, A[c][2]);
+                const float dr = p.r - (A_MU0 + c * A_DMU);
+                const float w  = __expf(-A_ETA * dr * dr) * p.fc;
+                A[c][0] = fmaf(w, p.ux, A[c][0]);
+                A[c][1] = fmaf(w, p.uy, A[c][1]);
+                A[c][2] = fmaf(w, p.uz, A[c][2]);
 #pragma unroll
-                for (int q = 0; q < 6; q++)
-                    T[c][q] = fmaf(w, uu[q], T[c][q]);
+                for (int m = 0; m < 6; m++)
+                    T[c][m] = fmaf(w, uu[m], T[c][m]);
             }
         }
+        nc += __popc(grp);
     }
 #pragma unroll
     for (int k = 0; k < NR; k++)
@@ -182,13 +247,14 @@ __global__ void __launch_bounds__(BLOCK) k_sur_desc(int nlocal, int i0, int n,
 #pragma unroll
     for (int c = 0; c < NA; c++) {
 #pragma unroll
-        for (int q = 0; q < 3; q++)
-            A[c][q] = gsum<LPA>(A[c][q]);
+        for (int m = 0; m < 3; m++)
+            A[c][m] = gsum<LPA>(A[c][m]);
 #pragma unroll
-        for (int q = 0; q < 6; q++)
-            T[c][q] = gsum<LPA>(T[c][q]);
+        for (int m = 0; m < 6; m++)
+            T[c][m] = gsum<LPA>(T[c][m]);
     }
-    if (!act || lane != 0) return;
+    if (!act || q != 0) return;
+    ncl[a] = nc;
     float D[ND];
 #pragma unroll
     for (int k = 0; k < NR; k++)
@@ -204,12 +270,12 @@ __global__ void __launch_bounds__(BLOCK) k_sur_desc(int nlocal, int i0, int n,
         D[NR + NA + c] = 1.5f * (T[c][0] * T[c][0] + T[c][1] * T[c][1] + T[c][2] * T[c][2] +
                                     2.0f * (T[c][3] * T[c][3] + T[c][4] * T[c][4] +
                                                T[c][5] * T[c][5]));
-        mo[3 * c]     = A[c][0];
-        mo[3 * c + 1] = A[c][1];
-        mo[3 * c + 2] = A[c][2];
+        mo[9 * c]     = A[c][0];
+        mo[9 * c + 1] = A[c][1];
+        mo[9 * c + 2] = A[c][2];
 #pragma unroll
-        for (int q = 0; q < 6; q++)
-            mo[12 + 6 * c + q] = T[c][q];
+        for (int m = 0; m < 6; m++)
+            mo[9 * c + 3 + m] = T[c][m];
     }
     float* o = dn_out + (size_t)a * ND;
 #pragma unroll
@@ -262,6 +328,13 @@ __global__ void k_sur_pack(const float* __restrict__ th, bf16x8* __restrict__ fr
     frag[f * 64 + l] = v;
 }
 
+// tanh(x) = 1 - 2 / (1 + exp(2x)): one exponential and one reciprocal (1 ulp); saturates at
+// both ends (exp overflow: rcp(inf) = 0 gives 1; underflow gives -1)
+__device__ inline float tanh_fast(float x)
+{
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * x));
+}
+
 __device__ inline bf16x8 pack8(const float* a, const float* b)
 {
     bf16x8 v;
@@ -282,7 +355,7 @@ __device__ inline floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c)
 __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restrict__ dn,
     const float* __restrict__ mom, const bf16x8* __restrict__ frag, const float* __restrict__ th,
     const float* __restrict__ norm, float s_e, float e0, float* __restrict__ e_out,
-    float* __restrict__ coef)
+    __half* __restrict__ coef)
 {
     __shared__ bf16x8 s_frag[NFRAG * 64];
     for (int k = threadIdx.x; k < NFRAG * 64; k += blockDim.x)
@@ -326,7 +399,7 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
             acc = mfma(s_frag[t * 64 + l], X, acc);
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                h1[t][i] = tanhf(acc[i] + b1[t][i]);
+                h1[t][i] = tanh_fast(acc[i] + b1[t][i]);
         }
         // layer 2
         const bf16x8 B0 = pack8(h1[0], h1[1]), B1 = pack8(h1[2], h1[3]);
@@ -337,7 +410,7 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
             acc = mfma(s_frag[(5 + 2 * t) * 64 + l], B1, acc);
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                h2[t][i] = tanhf(acc[i] + b2[t][i]);
+                h2[t][i] = tanh_fast(acc[i] + b2[t][i]);
         }
         // energy: w3 . h2 over the 64 features, summed over the four lane groups
         float part = 0.f;
@@ -378,150 +451,126 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
                 gd[t][i] = s_e * acc[i] * istd[t][i];  // dE/dD_d, d = 16 t + 4 g + i
         }
         if (!act) continue;
-        // force coefficients: [24 radial][4 x 3: 2 g_q1 A_c][4 x 6: 3 g_q2 T~_c][4 zero]
-        float* co       = coef + (size_t)atom * NCOEF;
+        // force coefficients in lane blocks (see the layout above): radial d -> block d / 6,
+        // slot d % 6; channel c: 2 g_q1 A_c at 16 c + 6, 3 g_q2 T~_c at 16 c + 9, zero at 16 c + 15
+        // written as FP16: the force pass gathers a neighbour's row per pair, and half the
+        // bytes is half its time; 11 significant bits keep the pair terms to 5e-4
+        __half* co      = coef + (size_t)atom * NCOEF;
         const float* mo = mom + (size_t)atom * NMOM;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            co[4 * g + i] = gd[0][i];
+        for (int i = 0; i < 4; i++) {
+            const int d = 4 * g + i;
+            co[16 * (d / RPL) + d % RPL] = __float2half(gd[0][i]);
+        }
         if (g < 2) {
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                co[16 + 4 * g + i] = gd[1][i];
+            for (int i = 0; i < 4; i++) {
+                const int d = 16 + 4 * g + i;
+                co[16 * (d / RPL) + d % RPL] = __float2half(gd[1][i]);
+            }
         } else if (g == 2) {
 #pragma unroll
             for (int c = 0; c < 4; c++)
 #pragma unroll
-                for (int q = 0; q < 3; q++)
-                    co[NR + 3 * c + q] = 2.0f * gd[1][c] * mo[3 * c + q];
+                for (int k = 0; k < 3; k++)
+                    co[16 * c + 6 + k] = __float2half(2.0f * gd[1][c] * mo[9 * c + k]);
         } else {
 #pragma unroll
-            for (int c = 0; c < 4; c++)
+            for (int c = 0; c < 4; c++) {
 #pragma unroll
-                for (int q = 0; q < 6; q++)
-                    co[NR + 12 + 6 * c + q] = 3.0f * gd[1][c] * mo[12 + 6 * c + q];
-#pragma unroll
-            for (int q = 60; q < 64; q++)
-                co[q] = 0.f;
+                for (int k = 0; k < 6; k++)
+                    co[16 * c + 9 + k] = __float2half(3.0f * gd[1][c] * mo[9 * c + 3 + k]);
+                co[16 * c + 15] = __float2half(0.f);
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------- forces
-// dE_i/dx_j (u = (x_j - x_i) / r) from atom i's angular coefficients, channel by channel:
-//   w'(u.a) u + (w/r)(a - (u.a) u) + w'(u.M u) u + (2 w/r)(M u - (u.M u) u)
-__device__ inline void ang_grad(const float* co, float ux, float uy, float uz, float dw, float wr,
-    float* s, float* vx, float* vy, float* vz)
+// F_i = sum_j (dE_i/dx_j - dE_j/dx_i).  With u = (x_j - x_i) / r, the radial parts combine to
+// (S_i + S_j) u with S = sum_n c_n G_n'(r); an angular channel of atom i contributes
+// dE_i/dx_j = w'(u.a + u.M u) u + (w/r)(a - (u.a) u) + (2 w/r)(M u - (u.M u) u), and atom j's
+// the same with -u, so the difference needs only da = a_i - a_j and sM = M_i + M_j:
+//   w'(u.da + u.sM u) u + (w/r)(da - (u.da) u) + (2 w/r)(sM u - (u.sM u) u).
+// 16 halves of a coefficient lane block as floats (two 16-byte loads)
+__device__ inline void load_block(const __half* __restrict__ p, float* c)
 {
-    const float ax = co[0], ay = co[1], az = co[2];
-    const float ua = ux * ax + uy * ay + uz * az;
-    *s += dw * ua;
-    *vx += wr * (ax - ua * ux);
-    *vy += wr * (ay - ua * uy);
-    *vz += wr * (az - ua * uz);
-}
-
-__device__ inline void sym_grad(const float* B, float ux, float uy, float uz, float dw, float wr,
-    float* s, float* vx, float* vy, float* vz)
-{
-    // M u with M = [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
-    const float mx  = B[0] * ux + B[3] * uy + B[4] * uz;
-    const float my  = B[3] * ux + B[1] * uy + B[5] * uz;
-    const float mz  = B[4] * ux + B[5] * uy + B[2] * uz;
-    const float uMu = ux * mx + uy * my + uz * mz;
-    *s += dw * uMu;
-    *vx += 2.0f * wr * (mx - uMu * ux);
-    *vy += 2.0f * wr * (my - uMu * uy);
-    *vz += 2.0f * wr * (mz - uMu * uz);
+    const uint4* v = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint4 w     = v[h];
+        const unsigned u[4] = { w.x, w.y, w.z, w.w };
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const __half2 hh = *reinterpret_cast<const __half2*>(&u[k]);
+            const float2 ff  = __half22float2(hh);
+            c[8 * h + 2 * k]     = ff.x;
+            c[8 * h + 2 * k + 1] = ff.y;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sur_force(int nlocal, int i0, int n,
-    const double* __restrict__ x, const int* __restrict__ border_map,
-    const int* __restrict__ neighbors, const int* __restrict__ numneigh,
-    const float* __restrict__ coef, double* __restrict__ f)
+    const double* __restrict__ x, const int* __restrict__ border_map, const int* __restrict__ cl,
+    const int* __restrict__ ncl, const __half* __restrict__ coef, double* __restrict__ f)
 {
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
     const int a    = gid / LPA;
-    const int lane = gid % LPA;
+    const int q    = gid % LPA;
     const bool act = a < n;
     const int i    = i0 + (act ? a : 0);
-    float ci[60];
-    {
-        const float4* r4 = reinterpret_cast<const float4*>(coef + (size_t)(act ? a : 0) * NCOEF);
-#pragma unroll
-        for (int q = 0; q < 15; q++) {
-            const float4 v = r4[q];
-            ci[4 * q] = v.x; ci[4 * q + 1] = v.y; ci[4 * q + 2] = v.z; ci[4 * q + 3] = v.w;
-        }
-    }
+    float ci[16];
+    load_block(coef + (size_t)(act ? a : 0) * NCOEF + 16 * q, ci);
+    const float amu = A_MU0 + q * A_DMU;
+    const float k0n = 2.0f * R_ETA * (R_MU0 + RPL * q * R_DMU);  // k_n = -2 eta r + k0n + n dk
     float fx = 0.f, fy = 0.f, fz = 0.f;
     if (act) {
         const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-        const int nn = numneigh[i];
-        for (int k = lane; k < nn; k += LPA) {
-            const int j    = neighbors[(size_t)k * nlocal + i];
-            const float dx = (float)(x[3 * j] - xi);
-            const float dy = (float)(x[3 * j + 1] - yi);
-            const float dz = (float)(x[3 * j + 2] - zi);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < RC2)) continue;
+        const int nn = ncl[a];
+        for (int k = 0; k < nn; k++) {
+            const int j = cl[(size_t)k * n + a];
+            Pair p;
+            if (!pair_geom(x, xi, yi, zi, j, &p)) continue;
             const int rj = coef_row(j, nlocal, border_map) - i0;
             if (rj < 0 || rj >= n) continue;  // owner outside the range: the caller's error
-            const float4* cj4 = reinterpret_cast<const float4*>(coef + (size_t)rj * NCOEF);
-            float cj[60];
-#pragma unroll
-            for (int q = 0; q < 15; q++) {
-                const float4 v = cj4[q];
-                cj[4 * q] = v.x; cj[4 * q + 1] = v.y; cj[4 * q + 2] = v.z; cj[4 * q + 3] = v.w;
-            }
-            const float r  = __builtin_sqrtf(r2);
-            const float ir = rcp_refined(r);
-            const float ux = dx * ir, uy = dy * ir, uz = dz * ir;
-            float fc, dfc;
-            cutoff_fn(r2, r, &fc, &dfc);
-            float g[NR];
-            gaussians(r, g);
-            // radial: (S_i + S_j) u with S = sum_n c_n G_n'(r), G_n' = g_n (fc k_n + dfc),
-            // k_n = -2 eta (r - mu_n)
-            float S     = 0.f;
-            float kn    = -2.0f * R_ETA * (r - R_MU0);
+            float cj[16];
+            load_block(coef + (size_t)rj * NCOEF + 16 * q, cj);
+            float g[RPL];
+            gaussians6(p.r, q, g);
+            // radial: S = sum_n (c_i + c_j)_n g_n (fc k_n + dfc)
+            float kn       = fmaf(-2.0f * R_ETA, p.r, k0n);  // -2 eta (r - mu_{6q})
             const float dk = 2.0f * R_ETA * R_DMU;
+            float S        = 0.f;
 #pragma unroll
-            for (int q = 0; q < NR; q++) {
-                S = fmaf((ci[q] + cj[q]) * g[q], fmaf(fc, kn, dfc), S);
+            for (int m = 0; m < RPL; m++) {
+                S = fmaf((ci[m] + cj[m]) * g[m], fmaf(p.fc, kn, p.dfc), S);
                 kn += dk;
             }
-            float vx = 0.f, vy = 0.f, vz = 0.f;  // v_ij - v_ji minus its radial part S u
-#pragma unroll
-            for (int c = 0; c < NA; c++) {
-                const float dr = r - (A_MU0 + c * A_DMU);
-                const float ex = __expf(-A_ETA * dr * dr);
-                const float w  = ex * fc;
-                const float dw = ex * fmaf(-2.0f * A_ETA * dr, fc, dfc);
-                const float wr = w * ir;
-                // i side, direction u
-                float si = 0.f, pix = 0.f, piy = 0.f, piz = 0.f;
-                ang_grad(ci + NR + 3 * c, ux, uy, uz, dw, wr, &si, &pix, &piy, &piz);
-                sym_grad(ci + NR + 12 + 6 * c, ux, uy, uz, dw, wr, &si, &pix, &piy, &piz);
-                // j side, direction -u
-                float sj = 0.f, pjx = 0.f, pjy = 0.f, pjz = 0.f;
-                ang_grad(cj + NR + 3 * c, -ux, -uy, -uz, dw, wr, &sj, &pjx, &pjy, &pjz);
-                sym_grad(cj + NR + 12 + 6 * c, -ux, -uy, -uz, dw, wr, &sj, &pjx, &pjy, &pjz);
-                // v_ij = si u + p_i ; v_ji = sj (-u) + p_j
-                S += si + sj;
-                vx += pix - pjx;
-                vy += piy - pjy;
-                vz += piz - pjz;
-            }
-            fx += S * ux + vx;
-            fy += S * uy + vy;
-            fz += S * uz + vz;
+            // angular channel q, both atoms at once
+            const float dr  = p.r - amu;
+            const float ex  = __expf(-A_ETA * dr * dr);
+            const float w   = ex * p.fc;
+            const float dw  = ex * fmaf(-2.0f * A_ETA * dr, p.fc, p.dfc);
+            const float wr  = w * p.ir;
+            const float dax = ci[6] - cj[6], day = ci[7] - cj[7], daz = ci[8] - cj[8];
+            const float sxx = ci[9] + cj[9], syy = ci[10] + cj[10], szz = ci[11] + cj[11];
+            const float sxy = ci[12] + cj[12], sxz = ci[13] + cj[13], syz = ci[14] + cj[14];
+            const float ua  = p.ux * dax + p.uy * day + p.uz * daz;
+            const float mx  = sxx * p.ux + sxy * p.uy + sxz * p.uz;
+            const float my  = sxy * p.ux + syy * p.uy + syz * p.uz;
+            const float mz  = sxz * p.ux + syz * p.uy + szz * p.uz;
+            const float uMu = p.ux * mx + p.uy * my + p.uz * mz;
+            const float sa  = S + dw * (ua + uMu);
+            const float w2  = 2.0f * wr;
+            fx += sa * p.ux + wr * (dax - ua * p.ux) + w2 * (mx - uMu * p.ux);
+            fy += sa * p.uy + wr * (day - ua * p.uy) + w2 * (my - uMu * p.uy);
+            fz += sa * p.uz + wr * (daz - ua * p.uz) + w2 * (mz - uMu * p.uz);
         }
     }
     fx = gsum<LPA>(fx);
     fy = gsum<LPA>(fy);
     fz = gsum<LPA>(fz);
-    if (act && lane == 0) {
+    if (act && q == 0) {
         f[3 * (size_t)a]     = fx;
         f[3 * (size_t)a + 1] = fy;
         f[3 * (size_t)a + 2] = fz;
@@ -569,88 +618,72 @@ __global__ void k_sur_loss(int n, const float* __restrict__ e, const double* __r
 
 // ---------------------------------------------------------------- training: force adjoint
 // U_i = scale * 2 sum_j J_ij^T e_ij, e_ij = dF_i - dF_owner(j): the derivative of atom i's
-// descriptors when neighbour j moves by e_ij.
+// descriptors when neighbour j moves by e_ij.  Lane q produces the lane's own descriptors
+// (radial 6 q .. 6 q + 5, q1_q, q2_q).
 __global__ void __launch_bounds__(BLOCK) k_sur_adjoint(int nlocal, int i0, int n,
-    const double* __restrict__ x, const int* __restrict__ border_map,
-    const int* __restrict__ neighbors, const int* __restrict__ numneigh,
-    const float* __restrict__ mom, const float* __restrict__ dF, float scale,
-    float* __restrict__ U)
+    const double* __restrict__ x, const int* __restrict__ border_map, const int* __restrict__ cl,
+    const int* __restrict__ ncl, const float* __restrict__ mom, const float* __restrict__ dF,
+    float scale, float* __restrict__ U)
 {
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
     const int a    = gid / LPA;
-    const int lane = gid % LPA;
+    const int q    = gid % LPA;
     const bool act = a < n;
     const int i    = i0 + (act ? a : 0);
-    float u[ND];
+    float u[RPL], u1 = 0.f, u2 = 0.f;
 #pragma unroll
-    for (int k = 0; k < ND; k++)
+    for (int k = 0; k < RPL; k++)
         u[k] = 0.f;
+    const float amu = A_MU0 + q * A_DMU;
     if (act) {
-        const float* mo = mom + (size_t)a * NMOM;
-        float Am[NMOM];
-#pragma unroll
-        for (int k = 0; k < NMOM; k++)
-            Am[k] = mo[k];
+        const float* mo = mom + (size_t)a * NMOM + 9 * q;
+        const float Ax = mo[0], Ay = mo[1], Az = mo[2];
+        const float Txx = mo[3], Tyy = mo[4], Tzz = mo[5], Txy = mo[6], Txz = mo[7], Tyz = mo[8];
         const float ex0 = dF[3 * (size_t)a], ey0 = dF[3 * (size_t)a + 1], ez0 = dF[3 * (size_t)a + 2];
         const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-        const int nn = numneigh[i];
-        for (int k = lane; k < nn; k += LPA) {
-            const int j    = neighbors[(size_t)k * nlocal + i];
-            const float dx = (float)(x[3 * j] - xi);
-            const float dy = (float)(x[3 * j + 1] - yi);
-            const float dz = (float)(x[3 * j + 2] - zi);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < RC2)) continue;
-            const int rj   = coef_row(j, nlocal, border_map) - i0;
+        const int nn = ncl[a];
+        for (int k = 0; k < nn; k++) {
+            const int j = cl[(size_t)k * n + a];
+            Pair p;
+            if (!pair_geom(x, xi, yi, zi, j, &p)) continue;
+            const int rj = coef_row(j, nlocal, border_map) - i0;
             if (rj < 0 || rj >= n) continue;  // owner outside the range: the caller's error
             const float ex = ex0 - dF[3 * (size_t)rj];
             const float ey = ey0 - dF[3 * (size_t)rj + 1];
             const float ez = ez0 - dF[3 * (size_t)rj + 2];
-            const float r  = __builtin_sqrtf(r2);
-            const float ir = rcp_refined(r);
-            const float ux = dx * ir, uy = dy * ir, uz = dz * ir;
-            const float ue = ux * ex + uy * ey + uz * ez;
-            float fc, dfc;
-            cutoff_fn(r2, r, &fc, &dfc);
-            float g[NR];
-            gaussians(r, g);
-            float kn       = -2.0f * R_ETA * (r - R_MU0);
+            const float ue = p.ux * ex + p.uy * ey + p.uz * ez;
+            float g[RPL];
+            gaussians6(p.r, q, g);
+            float kn       = -2.0f * R_ETA * (p.r - (R_MU0 + RPL * q * R_DMU));
             const float dk = 2.0f * R_ETA * R_DMU;
 #pragma unroll
-            for (int q = 0; q < NR; q++) {
-                u[q] = fmaf(g[q] * fmaf(fc, kn, dfc), ue, u[q]);
+            for (int m = 0; m < RPL; m++) {
+                u[m] = fmaf(g[m] * fmaf(p.fc, kn, p.dfc), ue, u[m]);
                 kn += dk;
             }
-#pragma unroll
-            for (int c = 0; c < NA; c++) {
-                const float drr = r - (A_MU0 + c * A_DMU);
-                const float exr = __expf(-A_ETA * drr * drr);
-                const float w   = exr * fc;
-                const float dw  = exr * fmaf(-2.0f * A_ETA * drr, fc, dfc);
-                const float wr  = w * ir;
-                const float* Ac = Am + 3 * c;
-                const float uA  = ux * Ac[0] + uy * Ac[1] + uz * Ac[2];
-                const float eA  = ex * Ac[0] + ey * Ac[1] + ez * Ac[2];
-                u[NR + c] += 2.0f * (dw * ue * uA + wr * (eA - ue * uA));
-                const float* B  = Am + 12 + 6 * c;
-                const float mx  = B[0] * ux + B[3] * uy + B[4] * uz;
-                const float my  = B[3] * ux + B[1] * uy + B[5] * uz;
-                const float mz  = B[4] * ux + B[5] * uy + B[2] * uz;
-                const float uTu = ux * mx + uy * my + uz * mz;
-                const float eTu = ex * mx + ey * my + ez * mz;
-                u[NR + NA + c] += 3.0f * (dw * uTu * ue + 2.0f * wr * (eTu - uTu * ue));
-            }
+            const float dr  = p.r - amu;
+            const float exr = __expf(-A_ETA * dr * dr);
+            const float w   = exr * p.fc;
+            const float dw  = exr * fmaf(-2.0f * A_ETA * dr, p.fc, p.dfc);
+            const float wr  = w * p.ir;
+            const float uA  = p.ux * Ax + p.uy * Ay + p.uz * Az;
+            const float eA  = ex * Ax + ey * Ay + ez * Az;
+            u1 += 2.0f * (dw * ue * uA + wr * (eA - ue * uA));
+            const float mx  = Txx * p.ux + Txy * p.uy + Txz * p.uz;
+            const float my  = Txy * p.ux + Tyy * p.uy + Tyz * p.uz;
+            const float mz  = Txz * p.ux + Tyz * p.uy + Tzz * p.uz;
+            const float uTu = p.ux * mx + p.uy * my + p.uz * mz;
+            const float eTu = ex * mx + ey * my + ez * mz;
+            u2 += 3.0f * (dw * uTu * ue + 2.0f * wr * (eTu - uTu * ue));
         }
     }
+    if (!act) return;
+    float* o = U + (size_t)a * ND;
 #pragma unroll
-    for (int k = 0; k < ND; k++)
-        u[k] = gsum<LPA>(u[k]);
-    if (act && lane == 0) {
-        float* o = U + (size_t)a * ND;
-#pragma unroll
-        for (int k = 0; k < ND; k++)
-            o[k] = 2.0f * scale * u[k];
-    }
+    for (int m = 0; m < RPL; m++)
+        o[RPL * q + m] = 2.0f * scale * u[m];
+    o[NR + q]      = 2.0f * scale * u1;
+    o[NR + NA + q] = 2.0f * scale * u2;
 }
 
 // ---------------------------------------------------------------- training: weight gradient
@@ -969,7 +1002,8 @@ int surGetParamsGFX950(float* theta_host)
     return mdgCopyFromDevice(theta_host, M.theta, NPARAM * sizeof(float));
 }
 
-int surDescriptorsGFX950(const MdgSurSystem* sys, int i0, int n, float* dn, float* mom)
+int surDescriptorsGFX950(const MdgSurSystem* sys, int i0, int n, float* dn, float* mom, int cap,
+    int* pairs, int* npairs)
 {
     if (!model_ok()) return -1;
     if (n <= 0) return 0;
@@ -979,12 +1013,12 @@ int surDescriptorsGFX950(const MdgSurSystem* sys, int i0, int n, float* dn, floa
     }
     TimedRegion tr(F_SURROGATE);
     k_sur_desc<<<blocks_for((long long)n * LPA, BLOCK), BLOCK, 0, stream()>>>(sys->nlocal, i0, n,
-        sys->x, sys->neighbors, sys->numneigh, M.norm, dn, mom);
+        sys->x, sys->neighbors, sys->numneigh, M.norm, dn, mom, cap, pairs, npairs);
     MDG_LAUNCH_CHECK("sur_desc");
     return failed() ? -1 : 0;
 }
 
-int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, float* coef)
+int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, void* coef)
 {
     if (!model_ok()) return -1;
     if (n <= 0) return 0;
@@ -993,12 +1027,13 @@ int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, float*
     int grid        = (ntile + 3) / 4;
     if (grid > 2048) grid = 2048;
     k_sur_mlp<<<grid, BLOCK, 0, stream()>>>(n, dn, mom, M.frag, M.theta, M.norm, M.s_e, M.e0, e,
-        coef);
+        (__half*)coef);
     MDG_LAUNCH_CHECK("sur_mlp");
     return failed() ? -1 : 0;
 }
 
-int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const float* coef, double* f)
+int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, const int* npairs,
+    const void* coef, double* f)
 {
     if (!model_ok()) return -1;
     if (n <= 0) return 0;
@@ -1008,7 +1043,7 @@ int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const float* coef, d
     }
     TimedRegion tr(F_SURROGATE);
     k_sur_force<<<blocks_for((long long)n * LPA, BLOCK), BLOCK, 0, stream()>>>(sys->nlocal, i0, n,
-        sys->x, sys->border_map, sys->neighbors, sys->numneigh, coef, f);
+        sys->x, sys->border_map, pairs, npairs, (const __half*)coef, f);
     MDG_LAUNCH_CHECK("sur_force");
     return failed() ? -1 : 0;
 }
@@ -1046,8 +1081,8 @@ int surLossTermsGFX950(int n, const float* e, const double* f, const double* e_r
     return failed() ? -1 : 0;
 }
 
-int surAdjointGFX950(const MdgSurSystem* sys, int i0, int n, const float* mom, const float* dF,
-    float scale, float* U)
+int surAdjointGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, const int* npairs,
+    const float* mom, const float* dF, float scale, float* U)
 {
     if (failed() || !stream()) return -1;
     if (n <= 0) return 0;
@@ -1056,7 +1091,7 @@ int surAdjointGFX950(const MdgSurSystem* sys, int i0, int n, const float* mom, c
         return -1;
     }
     k_sur_adjoint<<<blocks_for((long long)n * LPA, BLOCK), BLOCK, 0, stream()>>>(sys->nlocal, i0,
-        n, sys->x, sys->border_map, sys->neighbors, sys->numneigh, mom, dF, scale, U);
+        n, sys->x, sys->border_map, pairs, npairs, mom, dF, scale, U);
     MDG_LAUNCH_CHECK("sur_adjoint");
     return failed() ? -1 : 0;
 }
@@ -1124,13 +1159,16 @@ double computeForceSurrogateGFX950(Parameter* param, Atom* atom, Neighbor* neigh
     const int nl    = atom->Nlocal;
     grow_n(&M.dn, &M.cap_dn, (size_t)nl * ND + 4, 0);
     grow_n(&M.mom, &M.cap_mom, (size_t)nl * NMOM + 4, 0);
-    grow_n(&M.coef, &M.cap_coef, (size_t)nl * NCOEF + 4, 0);
+    grow_n(&M.coef, &M.cap_coef, (size_t)nl * NCOEF + 8, 0);
+    const int cap = neighbor->maxneighs;
+    grow_n(&M.pairs, &M.cap_pairs, (size_t)nl * cap + 4, 0);
+    grow_n(&M.npairs, &M.cap_npairs, (size_t)nl + 4, 0);
     if (failed()) return 0.0;
     MdgSurSystem sys{ nl, atom->Nghost, atom->d_atom.x, atom->d_atom.border_map,
         neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh };
-    surDescriptorsGFX950(&sys, 0, nl, M.dn, M.mom);
+    surDescriptorsGFX950(&sys, 0, nl, M.dn, M.mom, cap, M.pairs, M.npairs);
     surEvaluateGFX950(nl, M.dn, M.mom, nullptr, M.coef);
-    surForcesGFX950(&sys, 0, nl, M.coef, atom->d_atom.fx);
+    surForcesGFX950(&sys, 0, nl, M.pairs, M.npairs, M.coef, atom->d_atom.fx);
     return wtime() - t0;
 }
 

@@ -69,11 +69,11 @@ def lib():
     L.surSetParamsDeviceGFX950.argtypes = [vp]
     L.surGetParamsGFX950.argtypes = [vp]
     sp = C.POINTER(SurSystem)
-    L.surDescriptorsGFX950.argtypes = [sp, i, i, vp, vp]
+    L.surDescriptorsGFX950.argtypes = [sp, i, i, vp, vp, i, vp, vp]
     L.surEvaluateGFX950.argtypes = [i, vp, vp, vp, vp]
-    L.surForcesGFX950.argtypes = [sp, i, i, vp, vp]
+    L.surForcesGFX950.argtypes = [sp, i, i, vp, vp, vp, vp]
     L.surLossTermsGFX950.argtypes = [i, vp, vp, vp, vp, f, i, vp, vp, vp]
-    L.surAdjointGFX950.argtypes = [sp, i, i, vp, vp, f, vp]
+    L.surAdjointGFX950.argtypes = [sp, i, i, vp, vp, vp, vp, f, vp]
     L.surWeightGradGFX950.argtypes = [i, vp, vp, vp, vp]
     L.surAdamGFX950.argtypes = [vp, vp, vp, vp, i, f, f, f, f]
     L.surForceRmseGFX950.argtypes = [i, vp, vp]
@@ -265,13 +265,16 @@ class Model:
 
 
 class Workspace:
-    """Per-batch device arrays for up to n atoms."""
+    """Per-batch device arrays for up to n atoms whose lists hold at most cap entries."""
 
-    def __init__(self, n):
+    def __init__(self, n, cap):
         self.n = n
+        self.cap = int(cap)
         self.dn = Dev(n * ND, np.float32)
         self.mom = Dev(n * NMOM, np.float32)
-        self.coef = Dev(n * NCOEF, np.float32)
+        self.coef = Dev(n * NCOEF, np.float16)
+        self.pairs = Dev(n * self.cap, np.int32)
+        self.npairs = Dev(n, np.int32)
         self.e = Dev(n, np.float32)
         self.f = Dev(3 * n, np.float64)
         self.cE = Dev(n, np.float32)
@@ -283,9 +286,11 @@ def predict(sysobj, i0, n, ws):
     """Descriptors, network and forces of atoms i0 .. i0+n-1 into the workspace."""
     L = lib()
     s = C.byref(sysobj)
-    _check(L.surDescriptorsGFX950(s, i0, n, ws.dn.ptr, ws.mom.ptr), "descriptors")
+    _check(L.surDescriptorsGFX950(s, i0, n, ws.dn.ptr, ws.mom.ptr, ws.cap, ws.pairs.ptr,
+                                  ws.npairs.ptr), "descriptors")
     _check(L.surEvaluateGFX950(n, ws.dn.ptr, ws.mom.ptr, ws.e.ptr, ws.coef.ptr), "evaluate")
-    _check(L.surForcesGFX950(s, i0, n, ws.coef.ptr, ws.f.ptr), "forces")
+    _check(L.surForcesGFX950(s, i0, n, ws.pairs.ptr, ws.npairs.ptr, ws.coef.ptr, ws.f.ptr),
+           "forces")
 
 
 def descriptor_stats(packed, frames_sample, ws):
@@ -296,7 +301,8 @@ def descriptor_stats(packed, frames_sample, ws):
     for k in frames_sample:
         i0, i1 = packed.frame_range(k, k + 1)
         _check(lib().surDescriptorsGFX950(C.byref(packed.sys), i0, i1 - i0, ws.dn.ptr,
-                                          ws.mom.ptr), "descriptors")
+                                          ws.mom.ptr, ws.cap, ws.pairs.ptr, ws.npairs.ptr),
+               "descriptors")
         acc.append(ws.dn.get((i1 - i0) * ND).reshape(-1, ND))
     D = np.concatenate(acc).astype(np.float64)
     return D.mean(0), D.std(0) + 1e-6
@@ -310,7 +316,7 @@ class Trainer:
         self.wE, self.wF = wE, wF
         self.log = log or (lambda s: None)
         nmax = max(self._max_batch_atoms(train), self._max_batch_atoms(valid))
-        self.ws = Workspace(nmax)
+        self.ws = Workspace(nmax, max(train.maxn, valid.maxn))
         self.theta = Dev.of(model.theta, np.float32)
         self.m = Dev(NPARAM, np.float32)
         self.v = Dev(NPARAM, np.float32)
@@ -331,8 +337,8 @@ class Trainer:
         sums = np.zeros(2)
         _check(L.surLossTermsGFX950(n, ws.e.ptr, ws.f.ptr, pk.e_ref.at(i0), pk.f_ref.at(3 * i0),
                                     self.wE, n, ws.cE.ptr, ws.dF.ptr, _p(sums)), "loss terms")
-        _check(L.surAdjointGFX950(C.byref(pk.sys), i0, n, ws.mom.ptr, ws.dF.ptr,
-                                  self.wF / (3.0 * n), ws.U.ptr), "adjoint")
+        _check(L.surAdjointGFX950(C.byref(pk.sys), i0, n, ws.pairs.ptr, ws.npairs.ptr, ws.mom.ptr,
+                                  ws.dF.ptr, self.wF / (3.0 * n), ws.U.ptr), "adjoint")
         self.grad.zero()
         _check(L.surWeightGradGFX950(n, ws.dn.ptr, ws.U.ptr, ws.cE.ptr, self.grad.ptr), "wgrad")
         self.t += 1
@@ -378,6 +384,8 @@ class Trainer:
                 k += 1
             ve, vf = self.evaluate(self.valid)
             hist.append((ep, np.sqrt(el / nb), np.sqrt(fl / nb), ve, vf))
+            if ep % max(1, epochs // 15) and ep != epochs - 1:
+                continue
             self.log("epoch %3d  train E %.2f meV/atom F %.1f meV/A   valid E %.2f meV/atom "
                      "F %.1f meV/A  (%.1f s)" % (ep, 1e3 * hist[-1][1], 1e3 * hist[-1][2],
                                                   1e3 * ve, 1e3 * vf, time.perf_counter() - t0))
@@ -398,7 +406,7 @@ def train_surrogate(temps=None, per_temp=100, every=50, nx=10, epochs=30, batch_
     nval = max(1, int(round(valid_fraction * len(frames))))
     valid = PackedFrames([frames[k] for k in order[:nval]])
     train = PackedFrames([frames[k] for k in order[nval:]])
-    ws0 = Workspace(int(max(f["nlocal"] for f in frames)))
+    ws0 = Workspace(int(max(f["nlocal"] for f in frames)), train.maxn)
     d_mean, d_std = descriptor_stats(train, range(0, train.nframes, max(1, train.nframes // 64)),
                                      ws0)
     del ws0

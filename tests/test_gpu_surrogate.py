@@ -45,6 +45,17 @@ def _model(es, p):
     return es.Model(p.flat().astype(np.float32), p.d_mean, p.d_std, p.e0, p.s_e)
 
 
+def _lane_blocks(c):
+    """The oracle's coefficient rows [24 radial][4 x 3 a][4 x 6 M][4 zero] in the device's lane
+    blocks: block q = [radial 6q .. 6q+5, a_q, M_q, 0]."""
+    out = np.zeros_like(c)
+    for q in range(4):
+        out[:, 16 * q:16 * q + 6] = c[:, 6 * q:6 * q + 6]
+        out[:, 16 * q + 6:16 * q + 9] = c[:, 24 + 3 * q:24 + 3 * q + 3]
+        out[:, 16 * q + 9:16 * q + 15] = c[:, 36 + 6 * q:36 + 6 * q + 6]
+    return out
+
+
 def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
@@ -70,19 +81,26 @@ def test_descriptors_and_moments_match_oracle(frame):
     es = surrogate_module()
     p = _params_for(frame)
     pk = es.PackedFrames([frame])
-    ws = es.Workspace(pk.nlocal)
+    ws = es.Workspace(pk.nlocal, pk.maxn)
     _model(es, p).upload()
     n = pk.nlocal
-    assert es.lib().surDescriptorsGFX950(es.C.byref(pk.sys), 0, n, ws.dn.ptr, ws.mom.ptr) == 0
+    assert es.lib().surDescriptorsGFX950(es.C.byref(pk.sys), 0, n, ws.dn.ptr, ws.mom.ptr, ws.cap,
+                                         ws.pairs.ptr, ws.npairs.ptr) == 0
+    # the compact pair list is the Verlet list cut at rc, in list order
+    npairs = ws.npairs.get(n)
+    np.testing.assert_array_equal(npairs, frame["numneigh"])
+    pl = ws.pairs.get(n * ws.cap).reshape(ws.cap, n)
+    csr = np.concatenate([pl[:npairs[a], a] for a in range(n)])
+    np.testing.assert_array_equal(csr, frame["neighbors"])
     dn = ws.dn.get(n * so.ND).reshape(n, so.ND)
-    mom = ws.mom.get(n * 36).reshape(n, 36)
+    mom = ws.mom.get(n * 36).reshape(n, 4, 9)  # per channel: A (3), traceless T (6)
     D, A, T = so.descriptors(frame["x"], n, frame["numneigh"], frame["neighbors"])
     Dn = (D - p.d_mean) / p.d_std
     # radial terms are O(1); compare each descriptor against its own scale
     scale = np.maximum(np.abs(D).max(0), 1e-6) / p.d_std
     assert np.max(np.abs(dn - Dn) / scale) < 2e-5
-    np.testing.assert_allclose(mom[:, :12].reshape(n, 4, 3), A, atol=2e-5 * np.abs(A).max())
-    Tg = mom[:, 12:].reshape(n, 4, 6)
+    np.testing.assert_allclose(mom[:, :, :3], A, atol=2e-5 * np.abs(A).max())
+    Tg = mom[:, :, 3:]
     To = np.stack([T[:, :, a, b] for a, b in so.SYM], -1)
     np.testing.assert_allclose(Tg, To, atol=2e-5 * max(np.abs(To).max(), 1e-3))
 
@@ -91,12 +109,12 @@ def test_network_and_forces_match_oracle(frame):
     es = surrogate_module()
     p = _params_for(frame)
     pk = es.PackedFrames([frame])
-    ws = es.Workspace(pk.nlocal)
+    ws = es.Workspace(pk.nlocal, pk.maxn)
     _model(es, p).upload()
     n = pk.nlocal
     es.predict(pk.sys, 0, n, ws)
     e = ws.e.get(n)
-    coef = ws.coef.get(n * 64).reshape(n, 64)
+    coef = ws.coef.get(n * 64).reshape(n, 64).astype(np.float64)
     f = ws.f.get(3 * n).reshape(n, 3)
     args = (frame["x"], n, frame["owner"], frame["numneigh"], frame["neighbors"])
     D, A, T = so.descriptors(frame["x"], n, frame["numneigh"], frame["neighbors"])
@@ -104,6 +122,7 @@ def test_network_and_forces_match_oracle(frame):
     gb = so.mlp_input_grad(p, cache, bf16_inputs=True)
     cb = so.coefficients(gb, A, T)
     Fb = so.forces(cb, *args)
+    cb = _lane_blocks(cb)
     E64, F64 = so.energy_and_forces(p, *args)
     print("energy rel %.2e, coef rel %.2e, force rel (bf16 oracle) %.2e, (fp64) %.2e"
           % (_rel(e - p.e0, Eb - p.e0), _rel(coef, cb), _rel(f, Fb), _rel(f, F64)), flush=True)
@@ -121,7 +140,7 @@ def test_training_gradients_match_oracle(frame):
     p = _params_for(frame)
     pk = es.PackedFrames([frame])
     n = pk.nlocal
-    ws = es.Workspace(n)
+    ws = es.Workspace(n, pk.maxn)
     _model(es, p).upload()
     es.predict(pk.sys, 0, n, ws)
     wE, wF = 1.0, 10.0
@@ -136,8 +155,8 @@ def test_training_gradients_match_oracle(frame):
                                atol=1e-7 * np.abs(cE).max())
     np.testing.assert_allclose(dF.reshape(-1), f - frame["f_ref"], rtol=1e-6, atol=1e-6)
     assert abs(sums[1] - ((f - frame["f_ref"]) ** 2).sum()) < 1e-9 * sums[1]
-    assert L.surAdjointGFX950(es.C.byref(pk.sys), 0, n, ws.mom.ptr, ws.dF.ptr, wF / (3.0 * n),
-                              ws.U.ptr) == 0
+    assert L.surAdjointGFX950(es.C.byref(pk.sys), 0, n, ws.pairs.ptr, ws.npairs.ptr, ws.mom.ptr,
+                              ws.dF.ptr, wF / (3.0 * n), ws.U.ptr) == 0
     U = ws.U.get(n * 32).reshape(n, 32).astype(np.float64)
     D, A, T = so.descriptors(frame["x"], n, frame["numneigh"], frame["neighbors"])
     Uo = wF / (3.0 * n) * so.force_adjoint(frame["x"], n, frame["owner"], frame["numneigh"],
@@ -170,8 +189,11 @@ def test_adam_step_matches_oracle():
     dt, dg, dm, dv = (es.Dev.of(a, np.float32) for a in (th, gr, m, v))
     assert es.lib().surAdamGFX950(dt.ptr, dg.ptr, dm.ptr, dv.ptr, 7, 1e-3, 0.9, 0.999,
                                   1e-8) == 0
+    # the kernel's constants are the float32 values of 0.9, 0.999 and 1e-3
+    f32 = lambda z: float(np.float32(z))  # noqa: E731
     ref, mr, vr = so.adam_step(th.astype(np.float64), gr.astype(np.float64),
-                               m.astype(np.float64), v.astype(np.float64), 7, lr=1e-3)
+                               m.astype(np.float64), v.astype(np.float64), 7, lr=f32(1e-3),
+                               beta1=f32(0.9), beta2=f32(0.999), eps=f32(1e-8))
     np.testing.assert_allclose(dm.get(), mr, rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(dv.get(), vr, rtol=1e-6, atol=1e-9)
     np.testing.assert_allclose(dt.get(), ref, rtol=1e-6, atol=1e-6)
@@ -179,11 +201,11 @@ def test_adam_step_matches_oracle():
 
 @pytest.fixture(scope="module")
 def small_training():
-    """A short run: 3 temperatures x 8 snapshots of 6^3-cell boxes, 12 epochs."""
+    """A short run: 4 temperatures x 10 snapshots of 6^3-cell boxes, 150 epochs."""
     es = surrogate_module()
-    model, info, data = es.train_surrogate(temps=[500.0, 1100.0, 1700.0], per_temp=8, every=25,
-                                           nx=6, epochs=12, batch_frames=4, wE=1.0, wF=1.0,
-                                           log=lambda s: print(s, flush=True))
+    model, info, data = es.train_surrogate(temps=[400.0, 900.0, 1400.0, 1900.0], per_temp=10,
+                                           every=25, nx=6, epochs=150, batch_frames=2, wE=1.0,
+                                           wF=1.0, log=lambda s: print(s, flush=True))
     return model, info, data
 
 
