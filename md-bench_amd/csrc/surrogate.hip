@@ -358,10 +358,12 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
     __half* __restrict__ coef)
 {
     __shared__ bf16x8 s_frag[NFRAG * 64];
+    __shared__ __align__(16) __half s_co[BLOCK / 64][16 * NCOEF];  // a wavefront's 16 rows
     for (int k = threadIdx.x; k < NFRAG * 64; k += blockDim.x)
         s_frag[k] = frag[k];
     __syncthreads();
     const int l = threadIdx.x & 63, m = l & 15, g = l >> 4;
+    __half* wco = s_co[threadIdx.x >> 6];
     // per-lane constants: biases and output weights of the lane's features 16 t + 4 g + i
     float b1[4][4], b2[4][4], w3[4][4];
 #pragma unroll
@@ -450,13 +452,13 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
             for (int i = 0; i < 4; i++)
                 gd[t][i] = s_e * acc[i] * istd[t][i];  // dE/dD_d, d = 16 t + 4 g + i
         }
-        if (!act) continue;
-        // force coefficients in lane blocks (see the layout above): radial d -> block d / 6,
-        // slot d % 6; channel c: 2 g_q1 A_c at 16 c + 6, 3 g_q2 T~_c at 16 c + 9, zero at 16 c + 15
-        // written as FP16: the force pass gathers a neighbour's row per pair, and half the
-        // bytes is half its time; 11 significant bits keep the pair terms to 5e-4
-        __half* co      = coef + (size_t)atom * NCOEF;
-        const float* mo = mom + (size_t)atom * NMOM;
+        // force coefficients in lane blocks (see the layout above), staged in LDS and stored as
+        // whole rows: radial d -> block d / 6, slot d % 6; channel c: 2 g_q1 A_c at 16 c + 6,
+        // 3 g_q2 T~_c at 16 c + 9, zero at 16 c + 15.  FP16: the force pass gathers a
+        // neighbour's row per pair, and half the bytes is half its time; 11 significant bits keep
+        // the pair terms to 5e-4.
+        __half* co      = wco + m * NCOEF;
+        const float* mo = mom + (size_t)ar * NMOM;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int d = 4 * g + i;
@@ -483,6 +485,23 @@ __global__ void __launch_bounds__(BLOCK) k_sur_mlp(int n, const float* __restric
                 co[16 * c + 15] = __float2half(0.f);
             }
         }
+        // the wavefront's own LDS writes are complete before its reads (in-order LDS queue;
+        // the fence keeps the compiler from moving them)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 16 rows x 128 B = 128 chunks of 16 B: two per lane
+        const uint4* src = reinterpret_cast<const uint4*>(wco);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ch  = l + 64 * h;
+            const int row = ch >> 3;
+            if (tile * 16 + row < n)
+                reinterpret_cast<uint4*>(coef + (size_t)(tile * 16 + row) * NCOEF)[ch & 7] = src[ch];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 

@@ -394,7 +394,7 @@ class Trainer:
 
 
 def train_surrogate(temps=None, per_temp=100, every=50, nx=10, epochs=30, batch_frames=16,
-                    valid_fraction=0.1, seed=0, wE=1.0, wF=1.0, log=None):
+                    valid_fraction=0.1, seed=0, wE=1.0, wF=1.0, lr0=2e-3, lr1=5e-5, log=None):
     """Snapshots from plain EAM, a 90/10 split, Adam on the GPU.  Returns (model, info)."""
     log = log or (lambda s: None)
     temps = np.linspace(300.0, 2000.0, 20) if temps is None else temps
@@ -415,7 +415,7 @@ def train_surrogate(temps=None, per_temp=100, every=50, nx=10, epochs=30, batch_
     tr = Trainer(train, valid, model, batch_frames=batch_frames, wE=wE, wF=wF, log=log)
     e_init, f_init = tr.evaluate(valid)
     t1 = time.perf_counter()
-    hist = tr.fit(epochs, seed=seed)
+    hist = tr.fit(epochs, lr0=lr0, lr1=lr1, seed=seed)
     t_fit = time.perf_counter() - t1
     info = dict(frames=len(frames), train_frames=train.nframes, valid_frames=valid.nframes,
                 train_atoms=train.nlocal, capture_s=t_cap, fit_s=t_fit, epochs=epochs,

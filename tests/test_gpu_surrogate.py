@@ -201,11 +201,11 @@ def test_adam_step_matches_oracle():
 
 @pytest.fixture(scope="module")
 def small_training():
-    """A short run: 4 temperatures x 10 snapshots of 6^3-cell boxes, 150 epochs."""
+    """A short run: 4 temperatures x 10 snapshots of 6^3-cell boxes, 600 epochs (about 2 s)."""
     es = surrogate_module()
     model, info, data = es.train_surrogate(temps=[400.0, 900.0, 1400.0, 1900.0], per_temp=10,
-                                           every=25, nx=6, epochs=150, batch_frames=2, wE=1.0,
-                                           wF=1.0, log=lambda s: print(s, flush=True))
+                                           every=25, nx=6, epochs=600, batch_frames=2, wE=1.0,
+                                           wF=1.0, lr0=3e-3, log=lambda s: print(s, flush=True))
     return model, info, data
 
 
