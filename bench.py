@@ -20,6 +20,8 @@ Workloads (seeded fcc lattices, synthetic velocities):
              (8,388,608 atoms), rho 1.2, T 0.8, per-pair cutoff 2.5 sigma_ab.
   eam-surrogate-64  the learned EAM surrogate: trained on 2000 plain-EAM snapshots of 4000-atom
              boxes, then run on 64^3 Cu cells (1,048,576 atoms) beside plain EAM.
+  stub-seq, stub-fix, stub-rand  MD-Bench's synthetic-pattern force benchmark: 1M atoms on a
+             line, 76 neighbours in a sequential, fixed or random pattern, force kernel only.
   hbm-copy   a plain 256 MiB streaming kernel: the scale factor for the HBM byte counters.
 
 `roofline` covers the dominant kernel (the force kernel): algorithmic bytes per launch over its
@@ -476,6 +478,31 @@ def cpu_baseline_surrogate(model, frame, reps=3):
                       "same model (force call only)" % (frame["nlocal"], reps)}
 
 
+def bench_stub(args, mod, pattern="seq"):
+    """MD-Bench's synthetic-pattern benchmark (main-stub): 1,048,576 atoms on a line, 76
+    neighbours each in the given pattern, the FP64 LJ force kernel alone."""
+    natoms, nn = 1 << 20, 76
+    r = mod.stub_run(pattern, natoms, nn, 1, ntimes=max(args.steps // 5, 20), strict=0)
+    # MD-Bench's data-volume model (src/verletlist/main-stub.c:244-249, stats.c:56-59): per atom
+    # 6 doubles and (nneighs + 2) ints; per neighbour position gather 24 B.
+    bytes_call = natoms * (6 * 8.0 + (nn + 2) * 4.0) + natoms * nn * 24.0
+    achieved = bytes_call / (r["ms_per_call"] * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": r["atom_updates_per_s"], "unit": "atom-steps/s",
+        "n_gpus": 1, "steps": max(args.steps // 5, 20), "warmup": 1,
+        "ms_per_step": r["ms_per_call"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "stub-%s: synthetic neighbour pattern '%s', %d atoms x %d "
+                               "neighbours, force kernel only" % (pattern, pattern, natoms, nn),
+                   "atoms": natoms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_force_lj (fast, 4 lanes/atom)",
+                     "avg_launch_ms": r["ms_per_call"], "bytes_per_launch": bytes_call},
+    }
+    return res, None
+
+
 def bench_copy(args, mod):
     """Plain streaming kernel of known size, the scale for HBM counters: each launch of
     torch.mul(src, 1.0, out=dst) reads and writes 256 MiB (2^26 FP32 values each way)."""
@@ -519,6 +546,9 @@ WORKLOADS = {
     "eam-64": lambda a, m: bench_eam(a, m, nx=64),
     "ka-128": bench_ka,
     "eam-surrogate-64": bench_surrogate,
+    "stub-seq": lambda a, m: bench_stub(a, m, "seq"),
+    "stub-fix": lambda a, m: bench_stub(a, m, "fix"),
+    "stub-rand": lambda a, m: bench_stub(a, m, "rand"),
     "hbm-copy": bench_copy,
 }
 DEFAULT = "cp-lj-64"

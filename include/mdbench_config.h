@@ -38,6 +38,9 @@ typedef struct {
        this probability, type 0 otherwise (replaces rand() % ntypes) */
     double type1_fraction;
     int type_seed;
+    /* Verlet-list runs: atoms (positions, velocities, types and box) from MD-Bench's dump format
+       instead of the lattice, as its -i option does (no thermo adjustment); NULL = lattice */
+    const char* input_file;
 } mdb_config;
 
 /* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */
@@ -59,6 +62,14 @@ void mdbEamFree(mdb_eam_tables* t);
 /* Writes the synthetic Cu-like funcfl file (rho = exp(-3(r/2.556-1)) with a smooth cutoff,
    Morse-like Z, F = -sqrt(rho); nr = nrho = 2000, cutoff 4.95). */
 int mdbWriteFuncflSynthetic(const char* path);
+/* MD-Bench's dump format (src/verletlist/atom.c:529-656).  box = xlo xhi ylo yhi zlo zhi.
+   mdbWriteDmp writes 17 significant digits (exact round trip; 0 or < 0 on error); mdbReadDmp
+   reads the first frame into arrays of cap atoms and returns the atom count (< 0 on error;
+   -4: more than cap atoms), *ntypes = largest type + 1. */
+int mdbWriteDmp(const char* path, int natoms, const double* x, const double* v, const int* type,
+    const double* box, int timestep);
+int mdbReadDmp(const char* path, int cap, double* x, double* v, int* type, double* box,
+    int* ntypes);
 
 #ifdef __cplusplus
 }

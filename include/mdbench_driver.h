@@ -39,6 +39,13 @@ int mdbVLUseSurrogate(mdb_vl* s, int on, int judge_every, double tol);
 /* out[5]: judges run, last RMS, largest RMS, step of the fallback (-1: none), fallback flag */
 void mdbVLSurrogateStats(const mdb_vl* s, double* out);
 
+/* MD-Bench's synthetic-pattern force benchmark (src/verletlist/main-stub.c) on the device:
+   pattern 0 = seq, 1 = fix, 2 = rand; natoms on a line, nneighs per list replicated nreps times,
+   ntimes LJ force calls (full or half list, strict or fast kernel).  out[5]: seconds, atom
+   updates per second, ms per force call, f_x of the first and of the last atom. */
+int mdbStubRun(int pattern, int natoms, int nneighs, int nreps, int ntimes, int half, int strict,
+    double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,10 @@ void mdgfx950Synchronize(void);
 void initDeviceGFX950(Parameter* param, Atom* atom, Neighbor* neighbor);
 /* releases everything initDeviceGFX950 and later calls allocated (no reference counterpart) */
 void freeDeviceGFX950(Atom* atom, Neighbor* neighbor);
+/* Copies a host-built Verlet list (the reference's CPU layout, neighbors[i * maxneighs + k],
+   numneigh[i]) to the device list (memcpyToGPU of the list, src/common/device.h:30-36; used
+   where lists do not come from buildNeighborGFX950, e.g. the synthetic-pattern benchmark). */
+void uploadNeighborGFX950(Atom* atom, Neighbor* neighbor);
 /* replace copyDataToCUDADevice / copyDataFromCUDADevice (src/verletlist/forceCuda.cu:327-345):
    x, v, type of Nlocal atoms; From also copies f */
 void copyDataToGFX950Device(Parameter* param, Atom* atom);

@@ -90,6 +90,8 @@ struct CPState {
     // j-cluster bins
     int *d_jcnt = nullptr, *d_jstart = nullptr, *d_jlist = nullptr, *d_gcur = nullptr;
     size_t cap_jcnt = 0, cap_jstart = 0, cap_jlist = 0, cap_gcur = 0;
+    double* d_jkey = nullptr;  // bbminz of jlist[m], for the list build's column search
+    size_t cap_jkey = 0;
     int *d_gbin_start = nullptr, *d_gbin_list = nullptr;
     size_t cap_gbin_start = 0, cap_gbin_list = 0;
     // lists
@@ -544,7 +546,8 @@ __global__ void k_cp_ghost_scatter(int nghost, const int* __restrict__ gbin,
 __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
     const int* __restrict__ ncl, const int* __restrict__ cstart, const int* __restrict__ gstart,
     const int* __restrict__ glist, const double* __restrict__ cj_bbox,
-    const int* __restrict__ jstart, int* __restrict__ jlist, int* __restrict__ status)
+    const int* __restrict__ jstart, int* __restrict__ jlist, double* __restrict__ jkey,
+    int* __restrict__ status)
 {
     __shared__ int s_cj[JBIN_CAP];
     __shared__ double s_key[JBIN_CAP];
@@ -570,6 +573,7 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
         for (int f = 0; f < n; f++)
             r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
         jlist[out + r] = cj;
+        jkey[out + r]  = k;
     }
 }
 
@@ -596,12 +600,12 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 
 // First position m in [lo, hi) of a bbminz-sorted column whose key is >= v (strict: > v).
 template <bool STRICT>
-__device__ __forceinline__ int col_search(const int* __restrict__ jlist,
-    const double* __restrict__ cj_bbox, int lo, int hi, double v)
+__device__ __forceinline__ int col_search(const double* __restrict__ jkey, int lo, int hi,
+    double v)
 {
     while (lo < hi) {
         int mid  = (lo + hi) >> 1;
-        double k = cj_bbox[6 * jlist[mid] + 4];
+        double k = jkey[mid];
         if (STRICT ? (k <= v) : (k < v)) lo = mid + 1;
         else hi = mid;
     }
@@ -619,7 +623,8 @@ template <int M, int N>
 __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ cj_bbox,
-    const int* __restrict__ jstart, const int* __restrict__ jlist, const double* __restrict__ sx,
+    const int* __restrict__ jstart, const int* __restrict__ jlist,
+    const double* __restrict__ jkey, const double* __restrict__ sx,
     double cutneighsq, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
     int maxneighs, int* __restrict__ nb, int* __restrict__ numneigh, int* __restrict__ nmasked,
     int* __restrict__ status)
@@ -655,8 +660,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
                 atomicOr(status, ST_BIN_RANGE);
             } else {
                 const int beg = jstart[jb], end = jstart[jb + 1];
-                lo           = col_search<false>(jlist, cj_bbox, beg, end, zlo);
-                const int hi = col_search<true>(jlist, cj_bbox, lo, end, zhi);
+                lo           = col_search<false>(jkey, beg, end, zlo);
+                const int hi = col_search<true>(jkey, lo, end, zhi);
                 cnt          = hi - lo;
             }
         }
@@ -841,11 +846,97 @@ __device__ inline void two_sum_merge(float& s, float& c, float s2, float c2)
     c        = c + c2 + er;
 }
 
+// Pair set.  A pair is inside when its FP64 r^2 (the FP64 kernel's own expression) is below the
+// cutoff: a pair whose r differs from r_c by less than an FP32 rounding would otherwise be inside
+// for one kernel and outside for the other and change two forces by the full F(r_c).  The FP32
+// r^2 of the rounded displacement is within 3e-7 of it (relative), so it decides every pair
+// outside a +-2e-6 band around r_c^2; a wavefront evaluates the FP64 r^2 only when one of its
+// pairs falls into the band.  Full lists exclude the self pair by r^2 > 0 (the same slot, the
+// only pair at distance 0); padded slots and list entries past the end (the dummy j-cluster)
+// sit at far sentinel positions and never pass.
+constexpr float CUT_BAND = 2e-6f;
+
+struct JPos {
+    int cj, sj;
+    double x, y, z;
+};
+
 template <int N, bool HALF, bool EV, bool UNIFORM, bool COMP>
-__global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int ntypes, Tab32 tab,
-    const double* __restrict__ sx, const int* __restrict__ st, const int* __restrict__ nb,
-    const int* __restrict__ numneigh, const int* __restrict__ ci_natoms, int maxneighs,
-    float* __restrict__ f, double* __restrict__ ev)
+struct CPAcc {
+    float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, e = 0.f, w = 0.f;
+
+    __device__ __forceinline__ void pair(const JPos& J, double xi, double yi, double zi, int si,
+        int ci_cj0, int ncj, int a, int ti, int ntypes, const int* __restrict__ st,
+        const float (*s_tab)[MAXT2], const double* s_cut, float eps48u, float eps4u,
+        float sig6u, float clo_u, float chi_u, double cut2u, float* __restrict__ f)
+    {
+        const double dxd = xi - J.x;
+        const double dyd = yi - J.y;
+        const double dzd = zi - J.z;
+        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+        const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
+        double cut2 = cut2u;
+        if (!UNIFORM) {
+            const int tij = ti * ntypes + st[J.sj];
+            eps48         = s_tab[0][tij];
+            eps4          = s_tab[1][tij];
+            sig6          = s_tab[2][tij];
+            cut2          = s_cut[tij];
+            clo           = (float)cut2 * (1.0f - CUT_BAND);
+            chi           = (float)cut2 * (1.0f + CUT_BAND);
+        }
+        bool in         = rsq < clo;
+        const bool near = !in && rsq < chi;
+        if (__builtin_expect(__any(near), 0)) {
+            const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+            if (near) in = rsqd < cut2;
+        }
+        if (HALF) in = in && !(J.cj == ci_cj0 && !(si < J.sj));
+        else in = in && rsq > 0.0f;
+        float r2i       = __builtin_amdgcn_rcpf(rsq);
+        r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
+        const float sr6 = r2i * r2i * r2i * sig6;
+        float fpair     = eps48 * sr6 * (sr6 - 0.5f) * r2i;
+        fpair           = in ? fpair : 0.0f;
+        if (COMP) {
+            kahan_add(fx, cx, dx * fpair);
+            kahan_add(fy, cy, dy * fpair);
+            kahan_add(fz, cz, dz * fpair);
+        } else {
+            fx = fmaf(dx, fpair, fx);
+            fy = fmaf(dy, fpair, fy);
+            fz = fmaf(dz, fpair, fz);
+        }
+        if (EV) {
+            const float wgt = (HALF && J.cj < ncj) ? 1.0f : 0.5f;
+            e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
+            w += wgt * rsq * fpair;
+        }
+        if (HALF) {
+            // j-force: reduce over the four i-lanes that share this j-atom, one atomic per atom
+            float jx = -dx * fpair, jy = -dy * fpair, jz = -dz * fpair;
+            jx += __shfl_xor(jx, 1, 64);
+            jy += __shfl_xor(jy, 1, 64);
+            jz += __shfl_xor(jz, 1, 64);
+            jx += __shfl_xor(jx, 2, 64);
+            jy += __shfl_xor(jy, 2, 64);
+            jz += __shfl_xor(jz, 2, 64);
+            if (a == 0 && J.cj < ncj && (jx != 0.f || jy != 0.f || jz != 0.f)) {
+                atomicAdd(&f[3 * J.sj], jx);
+                atomicAdd(&f[3 * J.sj + 1], jy);
+                atomicAdd(&f[3 * J.sj + 2], jz);
+            }
+        }
+    }
+};
+
+template <int N, bool HALF, bool EV, bool UNIFORM, bool COMP>
+__global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int dummy_cj,
+    int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
+    const int* __restrict__ nb, const int* __restrict__ numneigh,
+    const int* __restrict__ ci_natoms, int maxneighs, float* __restrict__ f,
+    double* __restrict__ ev)
 {
     constexpr int CPI = 16 / N;  // j-clusters per iteration
     __shared__ float s_tab[3][MAXT2];
@@ -878,90 +969,45 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int nt
     const int nn     = __builtin_amdgcn_readfirstlane(numneigh[ci]);
     const int* lst   = nb + (size_t)ci * maxneighs;
     const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
-    const double cut2du = tab.cut2d[0];
-    float fx = 0.f, fy = 0.f, fz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    float e = 0.f, w = 0.f;
-    // The list of this i-cluster is held in registers, 64 entries at a time (one per lane), and
-    // handed out with cross-lane reads; entries past the end read as the i-cluster's own
-    // j-cluster, a valid address whose pairs are masked.  The positions of the next iteration
-    // are loaded before the current ones are used.
-    int ent = lane < nn ? lst[lane] : ci_cj0;
-    int cj  = __shfl(ent, g, 64);
-    int sj  = cj * N + jj;
-    double xj = sx[3 * sj], yj = sx[3 * sj + 1], zj = sx[3 * sj + 2];
-    for (int k0 = 0; k0 < nn; k0 += CPI) {
-        const int k      = k0 + g;
-        const bool valid = k < nn;
-        const int ccj = cj, csj = sj;
-        const double cxj = xj, cyj = yj, czj = zj;
-        const int kn = k0 + CPI;
-        if (kn < nn) {
-            if ((kn & 63) == 0) ent = kn + lane < nn ? lst[kn + lane] : ci_cj0;
-            const int kk = kn + g;
-            // the cross-lane read runs in every lane: a lane outside a branch supplies no data
-            const int nxt = __shfl(ent, kk & 63, 64);
-            cj            = kk < nn ? nxt : ci_cj0;
-            sj = cj * N + jj;
-            xj = sx[3 * sj];
-            yj = sx[3 * sj + 1];
-            zj = sx[3 * sj + 2];
-        }
-        // Displacement and r^2 in FP64, evaluated as the FP64 reference kernel evaluates them,
-        // so that the set of pairs inside the cutoff is exactly the reference's (a pair that
-        // crossed r_c on an FP32 rounding would change a force by the full F(r_c)).  Then one
-        // rounding each of r^2 and the displacement to FP32 for the force arithmetic.
-        const double dxd = xi - cxj;
-        const double dyd = yi - cyj;
-        const double dzd = zi - czj;
-        const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
-        const float rsq  = (float)rsqd;
-        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u;
-        double cut2d = cut2du;
-        if (!UNIFORM) {
-            const int tij = ti * ntypes + st[csj];
-            eps48         = s_tab[0][tij];
-            eps4          = s_tab[1][tij];
-            sig6          = s_tab[2][tij];
-            cut2d         = s_cut[tij];
-        }
-        const bool excl = (ccj == ci_cj0) && (HALF ? !(si < csj) : (si == csj));
-        const bool in   = valid && ilive && !excl && rsqd < (double)cut2d;
-        float r2i       = __builtin_amdgcn_rcpf(rsq);
-        r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
-        const float sr6 = r2i * r2i * r2i * sig6;
-        float fpair     = eps48 * sr6 * (sr6 - 0.5f) * r2i;
-        fpair           = in ? fpair : 0.0f;
-        if (COMP) {
-            kahan_add(fx, cx, dx * fpair);
-            kahan_add(fy, cy, dy * fpair);
-            kahan_add(fz, cz, dz * fpair);
-        } else {
-            fx = fmaf(dx, fpair, fx);
-            fy = fmaf(dy, fpair, fy);
-            fz = fmaf(dz, fpair, fz);
-        }
-        if (EV) {
-            const float wgt = (HALF && ccj < ncj) ? 1.0f : 0.5f;
-            e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
-            w += wgt * rsq * fpair;
-        }
-        if (HALF) {
-            // j-force: reduce over the four i-lanes that share this j-atom, one atomic per atom
-            float jx = -dx * fpair, jy = -dy * fpair, jz = -dz * fpair;
-            jx += __shfl_xor(jx, 1, 64);
-            jy += __shfl_xor(jy, 1, 64);
-            jz += __shfl_xor(jz, 1, 64);
-            jx += __shfl_xor(jx, 2, 64);
-            jy += __shfl_xor(jy, 2, 64);
-            jz += __shfl_xor(jz, 2, 64);
-            if (a == 0 && valid && ccj < ncj && (jx != 0.f || jy != 0.f || jz != 0.f)) {
-                atomicAdd(&f[3 * csj], jx);
-                atomicAdd(&f[3 * csj + 1], jy);
-                atomicAdd(&f[3 * csj + 2], jz);
-            }
+    const double cut2u = tab.cut2d[0];
+    const float clo_u  = (float)cut2u * (1.0f - CUT_BAND), chi_u = (float)cut2u * (1.0f + CUT_BAND);
+    CPAcc<N, HALF, EV, UNIFORM, COMP> acc;
+    // The list of this i-cluster is held in registers, 128 entries at a time (two per lane), and
+    // handed out with cross-lane reads (in every lane: a lane outside a branch supplies no data);
+    // entries past the end read as the dummy j-cluster.  Two register sets alternate: the
+    // positions of the next iteration load while the current ones are used.  Every load inside
+    // the loop is unconditional (a fetch past the chunk reads valid entries and is discarded),
+    // so the compiler's wait before a use waits only for that use's own loads.
+    auto use = [&](const JPos& J) {
+        acc.pair(J, xi, yi, zi, si, ci_cj0, ncj, a, ti, ntypes, st, s_tab, s_cut, eps48u, eps4u,
+            sig6u, clo_u, chi_u, cut2u, f);
+    };
+    for (int c0 = 0; c0 < nn; c0 += 128) {
+        const int e0   = c0 + lane < nn ? lst[c0 + lane] : dummy_cj;
+        const int e1   = c0 + 64 + lane < nn ? lst[c0 + 64 + lane] : dummy_cj;
+        const int cend = min(nn, c0 + 128);
+        auto fetch = [&](int kb, JPos& J) {
+            const int src = (kb & 64) ? e1 : e0;  // kb is wave-uniform
+            J.cj = __shfl(src, (kb + g) & 63, 64);
+            J.sj = J.cj * N + jj;
+            J.x  = sx[3 * J.sj];
+            J.y  = sx[3 * J.sj + 1];
+            J.z  = sx[3 * J.sj + 2];
+        };
+        JPos A, B;
+        fetch(c0, A);
+        for (int k0 = c0; k0 < cend;) {
+            fetch(k0 + CPI, B);
+            use(A);
+            k0 += CPI;
+            if (k0 >= cend) break;
+            fetch(k0 + CPI, A);
+            use(B);
+            k0 += CPI;
         }
     }
+    float fx = acc.fx, fy = acc.fy, fz = acc.fz, cx = acc.cx, cy = acc.cy, cz = acc.cz;
+    const float e = acc.e, w = acc.w;
     // reduce the 16 j-slot lanes of each i-atom (lane bits 2..5)
 #pragma unroll
     for (int o = 4; o < 64; o <<= 1) {
@@ -1160,8 +1206,8 @@ void freeDeviceClusterGFX950(void)
         C.d_abin_start, C.d_abin_atoms, C.d_ncl, C.d_cstart, C.d_ci_natoms, C.d_ci_bin,
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
-        C.d_jlist, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb, C.d_numneigh, C.d_nmasked,
-        C.d_red, C.d_ev, C.d_status };
+        C.d_jlist, C.d_jkey, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb, C.d_numneigh,
+        C.d_nmasked, C.d_red, C.d_ev, C.d_status };
     for (void* p : ptrs)
         dfree(p);
     if (C.h_status) (void)hipHostFree(C.h_status);
@@ -1451,6 +1497,7 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
     grow_n(&C.d_jcnt, &C.cap_jcnt, (size_t)mb + 1, 0);
     grow_n(&C.d_jstart, &C.cap_jstart, (size_t)mb + 1, 0);
     grow_n(&C.d_jlist, &C.cap_jlist, (size_t)C.ncj + ng + 1, 0);
+    grow_n(&C.d_jkey, &C.cap_jkey, (size_t)C.ncj + ng + 1, 0);
     if (failed()) return;
     fill_i32(C.d_status, 0, 1);
     fill_i32(C.d_gcur, 0, (size_t)mb + 1);
@@ -1463,7 +1510,7 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
     k_cp_ghost_scatter<<<blocks_for(ng, BLOCK), BLOCK, 0, s>>>(ng, C.d_gbin, C.d_gbin_start,
         C.d_gcur, C.d_gbin_list);
     k_cp_jbin_merge<<<mb, 64, 0, s>>>(C.M, C.N, C.ncj, C.d_ncl, C.d_cstart, C.d_gbin_start,
-        C.d_gbin_list, C.d_cj_bbox, C.d_jstart, C.d_jlist, C.d_status);
+        C.d_gbin_list, C.d_cj_bbox, C.d_jstart, C.d_jlist, C.d_jkey, C.d_status);
     MDG_LAUNCH_CHECK("cp_bin_j");
 }
 
@@ -1491,7 +1538,8 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
         C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_cj_bbox, C.d_jstart,      \
-        C.d_jlist, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs, C.d_nb, C.d_numneigh,     \
+        C.d_jlist, C.d_jkey, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs, C.d_nb,         \
+        C.d_numneigh,                                                                            \
         C.d_nmasked, C.d_status)
         if (C.M == 4 && C.N == 4) MDG_NB(4, 4);
         else if (C.M == 4 && C.N == 8) MDG_NB(4, 8);
@@ -1557,7 +1605,8 @@ double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neigh
         if (half) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
         unsigned grid = (blocks_for(nci, BLOCK / 64) + 7) & ~7u;  // multiple of 8, see kernel
 #define MDG_F32(NN, H, E, U, CO)                                                                 \
-    k_cp_force_f32<NN, H, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.ntypes, C.t32, C.d_sx,   \
+    k_cp_force_f32<NN, H, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj, C.ntypes,      \
+        C.t32, C.d_sx,                                                                            \
         C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms, C.maxneighs, C.d_sf32, C.d_ev)
 #define MDG_F32_U(NN, H, E)                                                                      \
     do {                                                                                         \

@@ -54,9 +54,6 @@ struct State {
     size_t cap_bins = 0, cap_binstart = 0;
     int* d_binatoms = nullptr;
     size_t cap_binatoms = 0;
-    // local atoms in bin order (the list build's thread order)
-    int *d_lflag = nullptr, *d_lpos = nullptr, *d_order = nullptr;
-    size_t cap_lflag = 0, cap_lpos = 0, cap_order = 0;
     bool binned = false;
     // per-atom EAM energies (training labels for the learned potential)
     double* d_eatom = nullptr;
@@ -280,67 +277,118 @@ __global__ void k_bin_sort(int mbins, const int* __restrict__ binstart, int* __r
 
 // ---------------------------------------------------------------- neighbour build
 // (neighbor.c:260-352): stencil bins in order, atoms in bin order, rsq <= cutneighsq.
-__global__ void k_local_flag(int nall, int nlocal, const int* __restrict__ binatoms,
-    int* __restrict__ flag)
-{
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < nall) flag[m] = binatoms[m] < nlocal;
-}
-
-__global__ void k_local_order(int nall, const int* __restrict__ binatoms,
-    const int* __restrict__ flag, const int* __restrict__ pos, int* __restrict__ order)
-{
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < nall && flag[m]) order[pos[m]] = binatoms[m];
-}
-
-// One thread per local atom, taken in bin order: the lanes of a wavefront sit in the same or
-// adjacent bins, walk the same stencil columns in step and read the same candidate positions.
-__global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const int* __restrict__ order,
-    const double* __restrict__ x,
-    const int* __restrict__ type, int ntypes, Tables tab, int uniform, Grid g,
+// One wavefront per bin.  The bin's local atoms (32 at a time, bin order) sit in LDS; the
+// candidates of the bin's stencil, flattened in the reference's loop order (stencil bins in order,
+// atoms in bin order), are tested 64 at a time, one per lane, against every local atom of the
+// bin, and a ballot gives each accepted candidate its place in that atom's list.  A lane's two
+// loads per candidate are independent of the previous candidate's, and lanes of a wavefront read
+// neighbouring entries of the bin arrays.
+__global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const double* __restrict__ x,
+    const int* __restrict__ type, int ntypes, Tables tab, int uniform, int mbins,
     const int* __restrict__ binstart, const int* __restrict__ binatoms,
     const int* __restrict__ stencil, int nstencil, int half, int maxneighs,
     int* __restrict__ neighbors, int* __restrict__ numneigh, int* __restrict__ status)
 {
     __shared__ int s_st[1024];
+    __shared__ double s_x[BLOCK / 64][32][3];
+    __shared__ int s_id[BLOCK / 64][32], s_t[BLOCK / 64][32];
     for (int k = threadIdx.x; k < nstencil; k += blockDim.x)
         s_st[k] = stencil[k];
     __syncthreads();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlocal) return;
-    const int i     = order[t];
-    if (i < 0) return;
-    const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
-    const int ibin  = coord2bin(g, xt, yt, zt);
-    const int ti    = type[i];
-    int n           = 0;
-    for (int s = 0; s < nstencil; s++) {
-        int jb = ibin + s_st[s];
-        if (jb < 0 || jb >= g.mbins) {
-            atomicOr(&status[0], ST_BIN_RANGE);
-            continue;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + w);
+    if (b >= mbins) return;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int beg = binstart[b], end = binstart[b + 1];
+    for (int q0 = beg; q0 < end; q0 += 32) {
+        // this chunk's local atoms, in bin order
+        const int m0 = q0 + lane;
+        int i0 = -1;
+        if (lane < 32 && m0 < end) i0 = binatoms[m0];
+        const bool isloc = i0 >= 0 && i0 < nlocal;
+        const unsigned long long lb = __ballot(isloc);
+        const int nq = __popcll(lb);
+        if (nq == 0) continue;
+        if (isloc) {
+            const int q  = __popcll(lb & lt);
+            s_id[w][q]   = i0;
+            s_t[w][q]    = type[i0];
+            s_x[w][q][0] = x[3 * i0];
+            s_x[w][q][1] = x[3 * i0 + 1];
+            s_x[w][q][2] = x[3 * i0 + 2];
         }
-        const int e = binstart[jb + 1];
-        for (int m = binstart[jb]; m < e; m++) {
-            int j = binatoms[m];
-            if (j == i) continue;
-            if (half && j < i) continue;
-            double dx  = xt - x[3 * j];
-            double dy  = yt - x[3 * j + 1];
-            double dz  = zt - x[3 * j + 2];
-            double rsq = dx * dx + dy * dy + dz * dz;
-            double cn  = uniform ? tab.cutn[0] : tab.cutn[ti * ntypes + type[j]];
-            if (rsq <= cn) {
-                if (n < maxneighs) neighbors[(size_t)n * nlocal + i] = j;
-                n++;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int cnt = 0;  // lane q < nq: length of atom q's list so far
+        for (int s0 = 0; s0 < nstencil; s0 += 64) {
+            const int sidx = s0 + lane;
+            int lo = 0, cc = 0;
+            if (sidx < nstencil) {
+                const int jb = b + s_st[sidx];
+                if (jb < 0 || jb >= mbins) {
+                    atomicOr(&status[0], ST_BIN_RANGE);
+                } else {
+                    lo = binstart[jb];
+                    cc = binstart[jb + 1] - lo;
+                }
+            }
+            int inc = cc;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += v;
+            }
+            const int total = __shfl(inc, 63, 64);
+            const int pre   = inc - cc;
+            for (int t0 = 0; t0 < total; t0 += 64) {
+                const int t = t0 + lane;
+                // largest column c with pre[c] <= t (cross-lane reads in every lane)
+                int c = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int pc = __shfl(pre, c + step, 64);
+                    if (pc <= t) c += step;
+                }
+                const int m     = __shfl(lo, c, 64) + (t - __shfl(pre, c, 64));
+                const bool cand = t < total;
+                int j = -1, tj = 0;
+                double xj = 0.0, yj = 0.0, zj = 0.0;
+                if (cand) {
+                    j  = binatoms[m];
+                    xj = x[3 * j];
+                    yj = x[3 * j + 1];
+                    zj = x[3 * j + 2];
+                    if (!uniform) tj = type[j];
+                }
+                for (int q = 0; q < nq; q++) {
+                    const int i     = s_id[w][q];
+                    const double dx = s_x[w][q][0] - xj;
+                    const double dy = s_x[w][q][1] - yj;
+                    const double dz = s_x[w][q][2] - zj;
+                    const double rsq = dx * dx + dy * dy + dz * dz;
+                    const double cn  = uniform ? tab.cutn[0] : tab.cutn[s_t[w][q] * ntypes + tj];
+                    const bool pass  = cand && j != i && !(half && j < i) && rsq <= cn;
+                    const unsigned long long bq = __ballot(pass);
+                    const int base = __builtin_amdgcn_readlane(cnt, q);
+                    if (pass) {
+                        const int pos = base + __popcll(bq & lt);
+                        if (pos < maxneighs) neighbors[(size_t)pos * nlocal + i] = j;
+                    }
+                    if (lane == q) cnt += __popcll(bq);
+                }
             }
         }
-    }
-    numneigh[i] = n;
-    if (n > maxneighs) {
-        atomicOr(&status[0], ST_NB_OVERFLOW);
-        atomicMax(&status[1], n);
+        if (lane < nq) {
+            numneigh[s_id[w][lane]] = cnt;
+            if (cnt > maxneighs) {
+                atomicOr(&status[0], ST_NB_OVERFLOW);
+                atomicMax(&status[1], cnt);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -724,9 +772,6 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
     dfree(S.d_binstart);
     dfree(S.d_binatoms);
     dfree(S.d_eatom);
-    dfree(S.d_lflag);
-    dfree(S.d_lpos);
-    dfree(S.d_order);
     dfree(S.d_gcnt);
     dfree(S.d_goff);
     dfree(S.d_pbc);
@@ -740,6 +785,30 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
     if (S.h_status) (void)hipHostFree(S.h_status);
     if (S.h_red) (void)hipHostFree(S.h_red);
     S = State();
+}
+
+void uploadNeighborGFX950(Atom* atom, Neighbor* neighbor)
+{
+    if (failed() || !S.ready) return;
+    const int nlocal = atom->Nlocal, mn = neighbor->maxneighs;
+    grow_n(&neighbor->d_neighbor.neighbors, &S.cap_nb, (size_t)nlocal * mn, 0);
+    grow_n(&neighbor->d_neighbor.numneigh, &S.cap_numneigh, (size_t)nlocal, 0);
+    if (failed()) return;
+    // the host list is AoS (neighbors[i * maxneighs + k]); the device list is SoA
+    int* soa = (int*)malloc((size_t)nlocal * mn * sizeof(int) + 4);
+    if (!soa) {
+        set_error(MDG_ERR_CAPACITY, "uploadNeighborGFX950: host buffer");
+        return;
+    }
+    for (int i = 0; i < nlocal; i++)
+        for (int k = 0; k < neighbor->numneigh[i]; k++)
+            soa[(size_t)k * nlocal + i] = neighbor->neighbors[(size_t)i * mn + k];
+    MDG_HIP(hipMemcpyAsync(neighbor->d_neighbor.neighbors, soa, (size_t)nlocal * mn * sizeof(int),
+        hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipMemcpyAsync(neighbor->d_neighbor.numneigh, neighbor->numneigh,
+        (size_t)nlocal * sizeof(int), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    free(soa);
 }
 
 void copyDataToGFX950Device(Parameter* param, Atom* atom)
@@ -884,18 +953,6 @@ void binatomsGFX950(Atom* atom)
     k_bin_scatter<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, S.d_bin_of, S.d_binstart,
         S.d_bincount, S.d_binatoms);
     k_bin_sort<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, S.d_binstart, S.d_binatoms);
-    // locals in bin order
-    const int nlocal = atom->Nlocal;
-    grow_n(&S.d_lflag, &S.cap_lflag, (size_t)nall + 1, 0);
-    grow_n(&S.d_lpos, &S.cap_lpos, (size_t)nall + 1, 0);
-    grow_n(&S.d_order, &S.cap_order, (size_t)nlocal + 1, 0);
-    if (failed()) return;
-    // an atom outside the grid is in no bin; its slot stays -1 and the build reports the error
-    fill_i32(S.d_order, -1, (size_t)nlocal);
-    k_local_flag<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, nlocal, S.d_binatoms, S.d_lflag);
-    exclusive_scan(S.d_lflag, S.d_lpos, nall, S.d_lpos + nall);
-    k_local_order<<<blocks_for(nall, BLOCK), BLOCK, 0, s>>>(nall, S.d_binatoms, S.d_lflag,
-        S.d_lpos, S.d_order);
     MDG_LAUNCH_CHECK("binatoms");
     S.binned = true;
 }
@@ -913,8 +970,8 @@ void buildNeighborGFX950(Atom* atom, Neighbor* neighbor)
         fill_i32(S.d_status, 0, 2);
         {
             TimedRegion tr(F_NEIGHBOR);
-            k_build_neighbor<<<blocks_for(nlocal, BLOCK), BLOCK, 0, stream()>>>(nlocal, S.d_order,
-                atom->d_atom.x, atom->d_atom.type, S.ntypes, S.tab, S.uniform ? 1 : 0, S.g,
+            k_build_neighbor<<<blocks_for(S.g.mbins, BLOCK / 64), BLOCK, 0, stream()>>>(nlocal,
+                atom->d_atom.x, atom->d_atom.type, S.ntypes, S.tab, S.uniform ? 1 : 0, S.g.mbins,
                 S.d_binstart, S.d_binatoms, S.d_stencil, S.nstencil, neighbor->half_neigh,
                 neighbor->maxneighs, neighbor->d_neighbor.neighbors,
                 neighbor->d_neighbor.numneigh, S.d_status);

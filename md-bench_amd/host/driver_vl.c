@@ -137,12 +137,24 @@ int mdbVLCreate(const mdb_config* cfg, mdb_vl** out)
     Neighbor* nb   = &s->neighbor;
     nb->maxneighs  = 100; /* initNeighbor, neighbor.c:75 */
     nb->half_neigh = p->half_neigh;
-    if (mdb_create_atoms(p, &s->atom)) {
-        mdbVLDestroy(s);
-        return MDG_ERR_CAPACITY;
+    if (cfg->input_file) {
+        /* readAtom (main.c:69-73): the file's atoms and box, no thermo adjustment; setupNeighbor
+           then sizes the bins from the box (neighbor.c:107-140) */
+        p->input_file = strdup(cfg->input_file);
+        if (!p->input_file || mdb_load_atoms(p, &s->atom, cfg->input_file)) {
+            fprintf(stderr, "mdbVLCreate: cannot read atoms from %s\n", cfg->input_file);
+            mdbVLDestroy(s);
+            return MDG_ERR_ARG;
+        }
+        mdb_setup_thermo(&s->thermo, p, s->atom.Natoms);
+    } else {
+        if (mdb_create_atoms(p, &s->atom)) {
+            mdbVLDestroy(s);
+            return MDG_ERR_CAPACITY;
+        }
+        mdb_setup_thermo(&s->thermo, p, s->atom.Natoms);
+        mdb_adjust_thermo(&s->thermo, p, &s->atom);
     }
-    mdb_setup_thermo(&s->thermo, p, s->atom.Natoms);
-    mdb_adjust_thermo(&s->thermo, p, &s->atom);
     int n = s->atom.Nlocal;
     s->x0 = malloc(sizeof(double) * 3 * n);
     s->v0 = malloc(sizeof(double) * 3 * n);
@@ -282,6 +294,7 @@ void mdbVLDestroy(mdb_vl* s)
     mdb_free_atom_host(&s->atom);
     free(s->param.epsilon_per_type);
     free(s->param.sigma_per_type);
+    free(s->param.input_file);
     mdbEamFree(&s->eam);
     free(s->x0);
     free(s->v0);

@@ -11,6 +11,8 @@ typedef struct {
 
 void mdb_init_parameter(Parameter* p);
 int mdb_create_atoms(Parameter* p, Atom* a);
+int mdb_load_atoms(Parameter* p, Atom* a, const char* path);
+double mdb_wtime(void);
 void mdb_setup_thermo(mdb_thermo* th, Parameter* p, int natoms);
 void mdb_adjust_thermo(const mdb_thermo* th, const Parameter* p, Atom* a);
 void mdb_free_atom_host(Atom* a);

@@ -11,6 +11,8 @@
  *   setupThermo/adjustThermo  src/common/thermo.c:35-58, 102-159
  */
 #include <math.h>
+#include <time.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -165,6 +167,57 @@ int mdb_create_atoms(Parameter* p, Atom* a)
     return 0;
 }
 
+/* readAtom_dmp (src/verletlist/atom.c:529-656): atoms, box and types from a dump file; the
+   box replaces x/y/zprd, ntypes becomes the largest type + 1. */
+int mdb_load_atoms(Parameter* p, Atom* a, const char* path)
+{
+    FILE* fp = fopen(path, "r");
+    if (!fp) return -1;
+    /* count for the allocation: the atom count line follows its item line */
+    char line[256];
+    int n = -1;
+    while (fgets(line, sizeof(line), fp)) {
+        if (strncmp(line, "ITEM: NUMBER OF ATOMS", 21) == 0) {
+            if (fgets(line, sizeof(line), fp)) n = atoi(line);
+            break;
+        }
+    }
+    fclose(fp);
+    if (n <= 0) return -3;
+    a->Natoms = n;
+    a->Nlocal = 0;
+    a->Nghost = 0;
+    a->Nmax   = n;
+    a->x      = calloc(3 * (size_t)n, sizeof(double));
+    a->vx     = calloc(3 * (size_t)n, sizeof(double));
+    a->fx     = calloc(3 * (size_t)n, sizeof(double));
+    a->type   = calloc((size_t)n, sizeof(int));
+    a->sqrt_epsilon = calloc((size_t)n, sizeof(double));
+    a->sigma3       = calloc((size_t)n, sizeof(double));
+    if (!a->x || !a->vx || !a->fx || !a->type || !a->sqrt_epsilon || !a->sigma3) return -2;
+    double box[6];
+    int nt    = 0;
+    int got   = mdbReadDmp(path, n, a->x, a->vx, a->type, box, &nt);
+    if (got != n) return got < 0 ? got : -3;
+    a->Nlocal = n;
+    p->xlo    = box[0];
+    p->xhi    = box[1];
+    p->ylo    = box[2];
+    p->yhi    = box[3];
+    p->zlo    = box[4];
+    p->zhi    = box[5];
+    p->xprd   = p->xhi - p->xlo;
+    p->yprd   = p->yhi - p->ylo;
+    p->zprd   = p->zhi - p->zlo;
+    a->ntypes = nt;
+    type_tables(p, a);
+    for (int i = 0; i < n; i++) {
+        a->sqrt_epsilon[i] = a->sqrt_epsilon_per_type[a->type[i]];
+        a->sigma3[i]       = a->sigma3_per_type[a->type[i]];
+    }
+    return 0;
+}
+
 void mdb_setup_thermo(mdb_thermo* th, Parameter* p, int natoms)
 {
     if (p->force_field == FF_EAM) {
@@ -202,4 +255,11 @@ void mdb_adjust_thermo(const mdb_thermo* th, const Parameter* p, Atom* a)
     double factor = sqrt(p->temp / t);
     for (int i = 0; i < 3 * a->Nlocal; i++)
         a->vx[i] *= factor;
+}
+
+double mdb_wtime(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }

@@ -32,7 +32,7 @@ class Config(C.Structure):
         ("compensated", C.c_int),
         ("pair_epsilon", C.POINTER(C.c_double)), ("pair_sigma", C.POINTER(C.c_double)),
         ("pair_cutoff", C.POINTER(C.c_double)), ("type1_fraction", C.c_double),
-        ("type_seed", C.c_int),
+        ("type_seed", C.c_int), ("input_file", C.c_char_p),
     ]
 
 
@@ -116,6 +116,9 @@ def lib():
     L.mdbMyrandom.argtypes = [C.POINTER(C.c_int)]
     L.mdbMyrandom.restype = C.c_double
     L.mdbWriteFuncflSynthetic.argtypes = [C.c_char_p]
+    L.mdbStubRun.argtypes = [C.c_int] * 7 + [_dp]
+    L.mdbWriteDmp.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _ip, _dp, C.c_int]
+    L.mdbReadDmp.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _ip, _dp, C.POINTER(C.c_int)]
     L.mdgfx950Status.restype = C.c_int
     L.mdgfx950StatusMessage.restype = C.c_char_p
     L.mdgfx950SetStrict.argtypes = [C.c_int]
@@ -181,8 +184,8 @@ def default_config(**kw):
             arr = (C.c_double * len(v))(*v)
             keep.append(arr)
             setattr(cfg, k, C.cast(arr, C.POINTER(C.c_double)))
-        elif k == "eam_file":
-            cfg.eam_file = v.encode() if v else None
+        elif k in ("eam_file", "input_file"):
+            setattr(cfg, k, v.encode() if v else None)
         else:
             setattr(cfg, k, v)
     cfg._keep = keep
@@ -394,6 +397,38 @@ def create_lattice(**kw):
     rc = lib().mdbCreateLattice(C.byref(cfg), x, v, t, n)
     assert rc >= 0, rc
     return x[:3 * rc], v[:3 * rc], t[:rc]
+
+
+STUB_PATTERNS = {"seq": 0, "fix": 1, "rand": 2}
+
+
+def stub_run(pattern, natoms, nneighs=76, nreps=1, ntimes=200, half=0, strict=0):
+    """MD-Bench's synthetic-pattern force benchmark on the device (include/mdbench_driver.h)."""
+    out = np.zeros(5)
+    rc = lib().mdbStubRun(STUB_PATTERNS[pattern], natoms, nneighs, nreps, ntimes, half, strict,
+                          out)
+    if rc:
+        raise RuntimeError("mdbStubRun failed (%d): %s" % (rc, status()[1]))
+    return dict(seconds=out[0], atom_updates_per_s=out[1], ms_per_call=out[2], f0=out[3],
+                flast=out[4])
+
+
+def write_dmp(path, x, v, types, box, timestep=0):
+    """Atoms in MD-Bench's dump format (box = xlo xhi ylo yhi zlo zhi)."""
+    x = np.ascontiguousarray(x, np.float64)
+    rc = lib().mdbWriteDmp(path.encode(), x.size // 3, x, np.ascontiguousarray(v, np.float64),
+                           np.ascontiguousarray(types, np.int32),
+                           np.ascontiguousarray(box, np.float64), timestep)
+    assert rc == 0, rc
+
+
+def read_dmp(path, cap):
+    x, v, t = np.zeros(3 * cap), np.zeros(3 * cap), np.zeros(cap, np.int32)
+    box = np.zeros(6)
+    nt = C.c_int()
+    n = lib().mdbReadDmp(path.encode(), cap, x, v, t, box, C.byref(nt))
+    assert n > 0, n
+    return x[:3 * n], v[:3 * n], t[:n], box, nt.value
 
 
 def gpu_available():

@@ -42,6 +42,8 @@ typedef struct {
 } orc_vl_grid;
 
 int orc_vl_setup_grid(orc_vl_grid* g, int nx, int ny, int nz, double lattice, double cutneigh);
+/* The grid MD-Bench sets up for atoms read from a file (box = xlo xhi ylo yhi zlo zhi). */
+int orc_vl_setup_grid_box(orc_vl_grid* g, const double* box, double cutneigh);
 int orc_vl_coord2bin(const orc_vl_grid* g, double x, double y, double z);
 /* Bins all nall atoms; bins is CSR (binstart[mbins+1], binatoms[nall]) in atom-index order. */
 void orc_vl_binatoms(const orc_vl_grid* g, int nall, const double* x, int* binstart, int* binatoms);
@@ -109,6 +111,12 @@ int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, do
     const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
     int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
     int reneigh_every, double mass, double* f, double* ksum);
+/* The same run for atoms read from a file: the box grid above, positions as given. */
+int orc_vl_run_box(const double* box, int nlocal, double* x, double* v, const int* type,
+    int ntypes, const double* epsilon, const double* sigma6, const double* cutforcesq,
+    const double* cutneighsq, double cutneigh, int half, int force_field, const orc_eam* eam,
+    double dt, double dtforce, int ntimes, int reneigh_every, double mass, double* f,
+    double* ksum);
 
 /* ---------------------------------------------------------------- cluster pair (oracle_cp.c) */
 typedef struct orc_cp orc_cp;

@@ -31,10 +31,11 @@ static double vl_bindist(const orc_vl_grid* g, int i, int j, int k)
 
 /* initNeighbor (neighbor.c:60-99, nbin = (int)(5/6 n)) + setupNeighbor for generated lattices
    (neighbor.c:101-258, input_file == NULL branch, full stencil). */
+static int grid_tail(orc_vl_grid* g, double cutneigh);
+
 int orc_vl_setup_grid(orc_vl_grid* g, int nx, int ny, int nz, double lattice, double cutneigh)
 {
     const double neighscale = 5.0 / 6.0;
-    const double FACTOR     = 0.999;
     memset(g, 0, sizeof(*g));
     g->xprd       = nx * lattice;
     g->yprd       = ny * lattice;
@@ -50,7 +51,35 @@ int orc_vl_setup_grid(orc_vl_grid* g, int nx, int ny, int nz, double lattice, do
     g->bininvx    = 1.0 / g->binsizex;
     g->bininvy    = 1.0 / g->binsizey;
     g->bininvz    = 1.0 / g->binsizez;
+    return grid_tail(g, cutneigh);
+}
 
+/* setupNeighbor with an input file (neighbor.c:107-140): bins of cutneigh / 2 over the file's
+   box, bininv = nbin / length. */
+int orc_vl_setup_grid_box(orc_vl_grid* g, const double* box, double cutneigh)
+{
+    memset(g, 0, sizeof(*g));
+    g->xprd       = box[1] - box[0];
+    g->yprd       = box[3] - box[2];
+    g->zprd       = box[5] - box[4];
+    g->cutneigh   = cutneigh;
+    g->cutneighsq = cutneigh * cutneigh;
+    g->binsizex = g->binsizey = g->binsizez = cutneigh * 0.5;
+    g->nbinx = (int)((box[1] - box[0]) / g->binsizex);
+    g->nbiny = (int)((box[3] - box[2]) / g->binsizey);
+    g->nbinz = (int)((box[5] - box[4]) / g->binsizez);
+    if (g->nbinx == 0) g->nbinx = 1;
+    if (g->nbiny == 0) g->nbiny = 1;
+    if (g->nbinz == 0) g->nbinz = 1;
+    g->bininvx = g->nbinx / (box[1] - box[0]);
+    g->bininvy = g->nbiny / (box[3] - box[2]);
+    g->bininvz = g->nbinz / (box[5] - box[4]);
+    return grid_tail(g, cutneigh);
+}
+
+static int grid_tail(orc_vl_grid* g, double cutneigh)
+{
+    const double FACTOR = 0.999;
     g->pad_x = (int)(cutneigh * g->bininvx);
     while (g->pad_x * g->binsizex < FACTOR * cutneigh)
         g->pad_x++;
@@ -328,6 +357,12 @@ double orc_kinetic_sum(int n, const double* v, double mass)
 }
 
 /* Timestep loop: src/verletlist/main.c:46-113 (setup after adjustThermo) and :371-419. */
+static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
+    const int* type, int ntypes, const double* epsilon, const double* sigma6,
+    const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
+    int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
+    int reneigh_every, double mass, double* f, double* ksum);
+
 int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
     const int* type, int ntypes, const double* epsilon, const double* sigma6,
     const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
@@ -336,6 +371,29 @@ int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, do
 {
     orc_vl_grid g;
     if (orc_vl_setup_grid(&g, nx, ny, nz, lattice, cutneigh)) return -1;
+    return run_on_grid(&g, nlocal, x, v, type, ntypes, epsilon, sigma6, cutforcesq, cutneighsq,
+        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum);
+}
+
+int orc_vl_run_box(const double* box, int nlocal, double* x, double* v, const int* type,
+    int ntypes, const double* epsilon, const double* sigma6, const double* cutforcesq,
+    const double* cutneighsq, double cutneigh, int half, int force_field, const orc_eam* eam,
+    double dt, double dtforce, int ntimes, int reneigh_every, double mass, double* f,
+    double* ksum)
+{
+    orc_vl_grid g;
+    if (orc_vl_setup_grid_box(&g, box, cutneigh)) return -1;
+    return run_on_grid(&g, nlocal, x, v, type, ntypes, epsilon, sigma6, cutforcesq, cutneighsq,
+        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum);
+}
+
+static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
+    const int* type, int ntypes, const double* epsilon, const double* sigma6,
+    const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
+    int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
+    int reneigh_every, double mass, double* f, double* ksum)
+{
+    const orc_vl_grid g = *gp;
     int gcap         = 27 * nlocal + 1024;
     int nmax         = nlocal + gcap;
     double* xa       = malloc(sizeof(double) * 3 * nmax);

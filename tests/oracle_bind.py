@@ -76,6 +76,9 @@ def lib():
                              C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
                              C.POINTER(Eam), C.c_double, C.c_double, C.c_int, C.c_int,
                              C.c_double, _dp, _dp]
+    L.orc_vl_run_box.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp,
+                                 C.c_double, C.c_int, C.c_int, C.POINTER(Eam), C.c_double,
+                                 C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp]
     _lib = L
     return L
 
@@ -203,6 +206,25 @@ def vl_run(nx, ny, nz, lattice, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutn
                           1 if eam is not None else 0,
                           C.byref(eam.eam) if eam is not None else None,
                           dt, dtforce, ntimes, reneigh, mass, f, ks)
+    assert rc == 0, rc
+    return xx, vv, f, ks
+
+
+def vl_run_box(box, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutneigh, half, ntimes,
+               reneigh, dt, dtforce, mass):
+    """The Verlet-list run of atoms read from a file (bins sized from the box)."""
+    nlocal = x.size // 3
+    xx = np.ascontiguousarray(x, np.float64).copy()
+    vv = np.ascontiguousarray(v, np.float64).copy()
+    f = np.zeros(3 * nlocal)
+    ks = np.zeros(ntimes + 1)
+    rc = lib().orc_vl_run_box(np.ascontiguousarray(box, np.float64), nlocal, xx, vv,
+                              np.ascontiguousarray(type_, np.int32), ntypes,
+                              np.ascontiguousarray(eps, np.float64),
+                              np.ascontiguousarray(sig6, np.float64),
+                              np.ascontiguousarray(cutf, np.float64),
+                              np.ascontiguousarray(cutn, np.float64), cutneigh, int(half), 0, None,
+                              dt, dtforce, ntimes, reneigh, mass, f, ks)
     assert rc == 0, rc
     return xx, vv, f, ks
 

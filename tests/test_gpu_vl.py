@@ -191,3 +191,67 @@ def test_eam_atom_energy_matches_oracle(strict):
         assert abs(e - ear.sum()) <= 1e-11 * abs(e)
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("strict", [1, 0])
+def test_run_from_dump_file_matches_oracle(strict, tmp_path):
+    """Atoms read from MD-Bench's dump format (bins sized from the file's box, as MD-Bench does
+    with -i): the strict run is bit-identical to the oracle's, the fast one within 1e-12."""
+    P = product()
+    lat = (4 / 0.8442) ** (1 / 3)
+    x, v, t = P.create_lattice(nx=5, ny=5, nz=5)
+    L = 5 * lat
+    rng = np.random.default_rng(3)
+    x = np.mod(x + rng.normal(0, 0.05, x.size), L)
+    path = str(tmp_path / "start.dmp")
+    box = [0.0, L, 0.0, L, 0.0, L]
+    P.write_dmp(path, x, v, t, box)
+    sim = P.VLSim(input_file=path, ntimes=40, reneigh_every=20, strict=strict)
+    try:
+        assert sim.atom.Nlocal == x.size // 3
+        ks = sim.run(40, trace=True)
+        st = sim.state(lists=False)
+    finally:
+        sim.close()
+    nt = int(t.max()) + 1
+    one = np.ones(nt * nt)
+    xo, vo, fo, kso = ob.vl_run_box(np.array(box), x, v, t, nt, one, one, one * 6.25,
+                                    one * 2.8 ** 2, 2.8, 0, 40, 20, 0.005, 0.0025, 1.0)
+    if strict:
+        # the kinetic sum is a parallel reduction on the device: equal to a few ulps only
+        np.testing.assert_allclose(ks, kso[1:], rtol=1e-14)
+        nl = x.size // 3
+        np.testing.assert_array_equal(st["x"][:3 * nl], xo[:3 * nl])
+        np.testing.assert_array_equal(st["v"], vo)
+        np.testing.assert_array_equal(st["f"], fo)
+    else:
+        assert np.max(np.abs(ks - kso[1:]) / kso[1:]) < 1e-12
+        assert _relerr(st["f"], fo) < 1e-10
+
+
+@pytest.mark.parametrize("pattern", ["seq", "fix"])
+def test_synthetic_pattern_benchmark_forces_match_oracle(pattern):
+    """The synthetic-pattern benchmark (MD-Bench's main-stub): atoms on a line, every listed pair
+    inside the cutoff; the strict kernel's forces on the first and last atom equal the oracle's
+    on the same lists."""
+    P = product()
+    n, nn, nr = 600, 76, 2
+    res = P.stub_run(pattern, n, nn, nr, ntimes=3, strict=1)
+    x = np.repeat(np.arange(n) * 1e-5, 3)
+    lists = []
+    for i in range(n):
+        if pattern == "seq":
+            li = [(i + 1 + k) % n for k in range(nn)]
+        else:
+            li = list(range(nn))
+        lists.append(li * nr)
+    numneigh = np.full(n, nn * nr, np.int32)
+    nb = np.array(lists, np.int32).reshape(-1)
+    types = np.zeros(n, np.int32)
+    one = np.ones(16)
+    f, _, _ = ob.force_lj(n, x, types, 4, one, one, one * 1e12, numneigh, nb, 0)
+    # (the fixed pattern lists atom 0 as its own neighbour, as MD-Bench's does: no finite force)
+    if pattern == "seq":
+        assert res["f0"] == f[0]
+    assert res["flast"] == f[3 * (n - 1)] and np.isfinite(res["flast"])
+    assert res["atom_updates_per_s"] > 0
