@@ -104,6 +104,7 @@ def bench_vl(args, mod):
         sim.run(args.warmup)
         L = mod.lib()
         L.mdgfx950Synchronize()
+        L.mdgfx950TimeOnly(b"force_lj")  # events around the reported kernel only
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -114,6 +115,7 @@ def bench_vl(args, mod):
         nl = C.c_longlong()
         L.mdgfx950KernelTime(b"force_lj", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
+        L.mdgfx950TimeOnly(None)
         st = sim.state()
         sum_nn = int(st["numneigh"].sum())
         nlocal = st["nlocal"]
@@ -168,6 +170,7 @@ def bench_cp(args, mod, N=4, half=0):
         sim.run(args.warmup)
         L = mod.lib()
         L.mdgfx950Synchronize()
+        L.mdgfx950TimeOnly(b"force_cp")  # events around the reported kernel only
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -177,14 +180,17 @@ def bench_cp(args, mod, N=4, half=0):
         nl = C.c_longlong()
         L.mdgfx950KernelTime(b"force_cp", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
+        L.mdgfx950TimeOnly(None)
         info = sim.info()
         pairs = int(sim.get("numneigh").astype(np.int64).sum())
         nci = info["nci"]
         elapsed = t1 - t0
-        # Algorithmic bytes of one FP32-mixed cluster-pair force launch: per i-cluster the list
-        # count and cluster size (8 B), 4 FP64 positions (96 B) and 4 FP32 forces (48 B); per
-        # cluster pair the j-cluster index (4 B) and N FP64 positions (24 N B).
-        bytes_launch = nci * (8.0 + 96.0 + 48.0) + pairs * (4.0 + 24.0 * N)
+        # Algorithmic bytes of one FP32-mixed cluster-pair force launch, the per-unit figures of
+        # the gather-bound model: per i-cluster the list count (4 B), 4 FP32 positions (48 B) and
+        # 4 FP32 forces (48 B); per cluster pair the j-cluster index (4 B) and N FP32 positions
+        # (12 N B); a half list adds the read-modify-write of N j-forces (24 N B) per pair.
+        # (The kernel gathers FP64 positions, 24 B each, to form FP64 displacements.)
+        bytes_launch = nci * (4.0 + 48.0 + 48.0) + pairs * (4.0 + 12.0 * N + (24.0 * N if half else 0.0))
         avg_ms = ms.value / max(nl.value, 1)
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         wl = "cp-lj-64%s%s" % ("-4x8" if N == 8 else "", "-half" if half else "")
@@ -201,7 +207,10 @@ def bench_cp(args, mod, N=4, half=0):
                        "j_clusters_per_i_cluster": pairs / nci},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "k_cp_force_f32 (one wavefront per i-cluster)",
+                         "kernel": ("k_cp_force_f32 (one wavefront per i-cluster, atomic "
+                                    "j-forces)" if half else
+                                    "k_cp_force_f32_full (four i-clusters per wavefront, one "
+                                    "j-atom per lane)"),
                          "avg_launch_ms": avg_ms, "launches": nl.value,
                          "bytes_per_launch": bytes_launch},
         }
@@ -239,6 +248,7 @@ def bench_eam(args, mod, nx=32):
         sim.run(args.warmup)
         L = mod.lib()
         L.mdgfx950Synchronize()
+        L.mdgfx950TimeOnly(b"force_eam")  # events around the reported kernel only
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -248,6 +258,7 @@ def bench_eam(args, mod, nx=32):
         nl = C.c_longlong()
         L.mdgfx950KernelTime(b"force_eam", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
+        L.mdgfx950TimeOnly(None)
         st = sim.state()
         sum_nn = int(st["numneigh"].sum())
         nlocal = st["nlocal"]
@@ -330,6 +341,7 @@ def bench_ka(args, mod, nx=128, N=4):
         sim.run(args.warmup)
         L = mod.lib()
         L.mdgfx950Synchronize()
+        L.mdgfx950TimeOnly(b"force_cp")  # events around the reported kernel only
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -339,12 +351,13 @@ def bench_ka(args, mod, nx=128, N=4):
         nl = C.c_longlong()
         L.mdgfx950KernelTime(b"force_cp", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
+        L.mdgfx950TimeOnly(None)
         info = sim.info()
         pairs = int(sim.get("numneigh").astype(np.int64).sum())
         nci = info["nci"]
         elapsed = t1 - t0
-        # as for cp-lj-64, plus a 4-byte type per atom: i (16 B per cluster) and j (4 N B per pair)
-        bytes_launch = nci * (8.0 + 96.0 + 48.0 + 16.0) + pairs * (4.0 + 28.0 * N)
+        # as for cp-lj-64, plus a 1-byte type per atom: i (4 B per cluster) and j (N B per pair)
+        bytes_launch = nci * (4.0 + 48.0 + 48.0 + 4.0) + pairs * (4.0 + 13.0 * N)
         avg_ms = ms.value / max(nl.value, 1)
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         res = {
@@ -360,7 +373,7 @@ def bench_ka(args, mod, nx=128, N=4):
                        "j_clusters_per_i_cluster": pairs / nci},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "k_cp_force_f32 (type-pair table in LDS)",
+                         "kernel": "k_cp_force_f32_full (type-pair table in LDS)",
                          "avg_launch_ms": avg_ms, "launches": nl.value,
                          "bytes_per_launch": bytes_launch},
         }
@@ -403,6 +416,7 @@ def bench_surrogate(args, mod, nx=64):
         es.use_in(sim, model, judge_every=50, tol=0.030)
         sim.run(args.warmup)
         L.mdgfx950Synchronize()
+        L.mdgfx950TimeOnly(b"surrogate")  # events around the reported kernel only
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -412,6 +426,7 @@ def bench_surrogate(args, mod, nx=64):
         nl = C.c_longlong()
         L.mdgfx950KernelTime(b"surrogate", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
+        L.mdgfx950TimeOnly(None)
         judge = es.judge_stats(sim)
         elapsed = t1 - t0
         value = natoms * args.steps / elapsed

@@ -41,6 +41,9 @@ typedef struct {
     /* Verlet-list runs: atoms (positions, velocities, types and box) from MD-Bench's dump format
        instead of the lattice, as its -i option does (no thermo adjustment); NULL = lattice */
     const char* input_file;
+    /* cluster-pair runs: prune the lists every this many steps between rebuilds (pruneNeighbor,
+       src/clusterpair/main.c:329-332); 0 = the reference's default, 1000 */
+    int prune_every;
 } mdb_config;
 
 /* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */

@@ -45,8 +45,12 @@ void mdgfx950ClearStatus(void);
 void mdgfx950SetStrict(int strict);
 void mdgfx950EnableTiming(int on);
 /* Sum of kernel time (ms) and launch count of one kernel family since timing was enabled.
-   Families: "force_lj", "force_eam", "neighbor", "binning", "pbc", "integrate", "thermo". */
+   Families: "force_lj", "force_eam", "neighbor", "binning", "pbc", "integrate", "thermo",
+   "force_cp", "cluster", "surrogate". */
 int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches);
+/* Restricts event recording to one family (NULL: all families, the default), so that timing a
+   kernel adds no events around the others; returns -1 for an unknown family. */
+int mdgfx950TimeOnly(const char* family);
 /* Wait for all queued work. */
 void mdgfx950Synchronize(void);
 

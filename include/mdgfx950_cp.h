@@ -55,6 +55,11 @@ void setupPbcClusterGFX950(Atom* atom, Parameter* param);
 void binJClustersGFX950(Parameter* param, Atom* atom);
 /* replaces buildNeighborCPU (neighbor.c:282-766); grows neighbor->maxneighs on overflow */
 void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor);
+/* replaces pruneNeighborCPU (neighbor.c:1099-1264; PruneNeighborFunction, neighbor.h:57):
+   drops list entries with no atom pair inside cutneigh, in the reference's order (an entry out
+   of range is overwritten by the current last entry); main.c:329-332 calls it every
+   prune_every steps between rebuilds, before the ghost update */
+void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor);
 /* replace updatePbcCUDA / updatePbcCPU (force_lj_cuda.cu:583-611, pbc.c:43-116) and
    updateAtomsPbcCPU (pbc.c:120-145) */
 void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate);
@@ -65,6 +70,12 @@ void updateSingleAtomsGFX950(Parameter* param, Atom* atom);
    computeForceLJ4xn{Full,Half}Neigh (src/clusterpair/force_lj.c): full or half list by
    neighbor->half_neigh, precision by setClusterPrecisionGFX950 */
 double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
+/* computeForce then finalIntegrate (main.c:349-350) in one call: the FP32 full-list kernel
+   applies the final velocity kick to each atom whose total force it has just written (one pass
+   fewer over v and f); other configurations run the two calls in order (no reference
+   counterpart: a fused form of the two reference calls) */
+double computeForceFinalClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats);
 /* replace cudaInitialIntegrate / cudaFinalIntegrate (force_lj_cuda.cu:456-542; this one uses
    dtforce in the final kick, where the reference's CUDA final kick uses dt) */
 void initialIntegrateClusterGFX950(Parameter* param, Atom* atom);
@@ -79,6 +90,9 @@ void energyVirialClusterGFX950(double* epot, double* virial);
    ci_bin, ci_bbox (6/ci), cj_natoms, cj_bbox, ghost_map, ghost_pbc, numneigh,
    numneigh_masked, neighbors (CSR), cbin_start, cbin_list, atoms_x, atoms_v, atoms_type. */
 long long downloadClusterArrayGFX950(const char* name, void* out, long long cap);
+/* Overwrites the first n values of a named device array from host (test hook; names: slot_x,
+   slot_v).  Returns n, -1 unknown name, -2 n larger than the array. */
+long long uploadClusterArrayGFX950(const char* name, const void* in, long long n);
 /* nci ncj nghost_clusters dummy_cj nghost_atoms mbins nstencil maxneighs M N natoms */
 void clusterInfoGFX950(int* info);
 

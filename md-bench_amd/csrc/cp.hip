@@ -49,7 +49,8 @@ struct Tab64 {
     double eps[MAXT2], sig6[MAXT2], cutf[MAXT2];
 };
 struct Tab32 {
-    float eps48[MAXT2], eps4[MAXT2], sig6[MAXT2], cut2[MAXT2];
+    // F/r = r^-8 (c12 r^-6 - c6): c12 = 48 eps sigma^12, c6 = 24 eps sigma^6
+    float c12[MAXT2], c6[MAXT2], eps4[MAXT2], sig6[MAXT2], cut2[MAXT2];
     double cut2d[MAXT2];  // cutoffs for the FP64 inclusion test
 };
 
@@ -92,6 +93,9 @@ struct CPState {
     size_t cap_jcnt = 0, cap_jstart = 0, cap_jlist = 0, cap_gcur = 0;
     double* d_jkey = nullptr;  // bbminz of jlist[m], for the list build's column search
     size_t cap_jkey = 0;
+    int* d_zidx = nullptr;     // per column, first jlist position at or above each z boundary
+    size_t cap_zidx = 0;
+    double z0 = 0, inv_dz = 1;
     int *d_gbin_start = nullptr, *d_gbin_list = nullptr;
     size_t cap_gbin_start = 0, cap_gbin_list = 0;
     // lists
@@ -200,130 +204,169 @@ __global__ void k_cp_wrap(int n3, double* __restrict__ x, double xprd, double yp
 }
 
 // ---------------------------------------------------------------- binAtoms (neighbor.c:1409-1449)
+// Atoms arrive in cluster order after the first build, so a wavefront's atoms mostly share one
+// column: the lanes of each distinct bin are counted with a ballot and one lane adds the count
+// (64 lanes hitting one counter would serialise).  Every lane runs the loop; it is wave-uniform.
 __global__ void k_cp_bin_count(int natoms, const double* __restrict__ ax, CGrid g,
     int* __restrict__ bin_of, int* __restrict__ cnt, int* __restrict__ status)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    int ix, iy;
-    c2b2d(g, ax[3 * i], ax[3 * i + 1], &ix, &iy);
-    int b = iy * g.mbinx + ix + 1;
-    if (b < 0 || b >= g.mbins) {
-        atomicOr(status, ST_BIN_RANGE);
-        bin_of[i] = -1;
-        return;
+    const int i    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int b          = -1;
+    if (i < natoms) {
+        int ix, iy;
+        c2b2d(g, ax[3 * i], ax[3 * i + 1], &ix, &iy);
+        b = iy * g.mbinx + ix + 1;
+        if (b < 0 || b >= g.mbins) {
+            atomicOr(status, ST_BIN_RANGE);
+            b = -1;
+        }
+        bin_of[i] = b;
     }
-    bin_of[i] = b;
-    atomicAdd(&cnt[b], 1);
+    unsigned long long act = __ballot(b >= 0);
+    while (act) {
+        const int l                  = __ffsll((long long)act) - 1;
+        const int lb                 = __shfl(b, l, 64);
+        const unsigned long long grp = __ballot(b == lb);
+        if (lane == l) atomicAdd(&cnt[lb], __popcll(grp));
+        act &= ~grp;
+    }
 }
 
+// Same grouping; lanes of one bin take consecutive places in lane (atom-index) order.
 __global__ void k_cp_bin_scatter(int natoms, const int* __restrict__ bin_of,
     const int* __restrict__ start, int* __restrict__ cursor, int* __restrict__ atoms)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    int b = bin_of[i];
-    if (b < 0) return;
-    atoms[start[b] + atomicAdd(&cursor[b], 1)] = i;
+    const int i    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int b    = i < natoms ? bin_of[i] : -1;
+    unsigned long long act = __ballot(b >= 0);
+    while (act) {
+        const int l                  = __ffsll((long long)act) - 1;
+        const int lb                 = __shfl(b, l, 64);
+        const unsigned long long grp = __ballot(b == lb);
+        int base                     = 0;
+        if (lane == l) base = atomicAdd(&cursor[lb], __popcll(grp));
+        base = __shfl(base, l, 64);
+        if (b == lb) atoms[start[lb] + base + __popcll(grp & lt)] = i;
+        act &= ~grp;
+    }
 }
 
-// One wavefront per bin: restore atom-index order (the reference fills bins in index order),
-// then replay sortAtomsByZCoord (neighbor.c:1452-1481): at step a, the first position holding
-// the minimum z among positions a..c-1 swaps with position a.  Also counts the bin's clusters.
-__global__ void __launch_bounds__(64) k_cp_bin_sort(const int* __restrict__ start,
+// One block per bin.  The reference fills a bin in atom-index order and then runs
+// sortAtomsByZCoord (neighbor.c:1452-1481): at step a, the first position holding the minimum z
+// among positions a..c-1 swaps with position a.  With all z distinct that selection sort ends in
+// the unique z order, so each atom's place is its z rank.  Only a column with equal z values (a
+// perfect lattice) restores index order and replays the selection sort, whose swaps decide the
+// order of ties.  Also counts the bin's clusters.
+constexpr int SORT_T = 256;
+__global__ void __launch_bounds__(SORT_T) k_cp_bin_sort(const int* __restrict__ start,
     int* __restrict__ atoms, const double* __restrict__ ax, int M, int N, int* __restrict__ ncl,
     int* __restrict__ status)
 {
-    __shared__ int s_idx[BIN_CAP];
-    __shared__ int s_tmp[BIN_CAP];
-    __shared__ double s_z[BIN_CAP];
-    const int b = blockIdx.x, lane = threadIdx.x;
+    __shared__ __align__(16) int s_idx[BIN_CAP];
+    __shared__ __align__(16) int s_tmp[BIN_CAP];
+    __shared__ __align__(16) double s_z[BIN_CAP];
+    __shared__ int s_tie;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
     const int s0 = start[b], c = start[b + 1] - s0;
     if (c > BIN_CAP) {
-        if (lane == 0) {
+        if (t == 0) {
             atomicOr(status, ST_BIN_CAP);
             ncl[b] = 0;
         }
         return;
     }
-    for (int p = lane; p < c; p += 64)
-        s_idx[p] = atoms[s0 + p];
-    __syncthreads();
-    for (int p = lane; p < c; p += 64) {
-        int v = s_idx[p], r = 0;
-        for (int q = 0; q < c; q++)
-            r += s_idx[q] < v;
-        s_tmp[r] = v;
-    }
-    __syncthreads();
-    for (int p = lane; p < c; p += 64) {
-        s_idx[p] = s_tmp[p];
-        s_z[p]   = ax[3 * s_tmp[p] + 2];
-    }
-    __syncthreads();
-    // With all z distinct the selection sort's result is the unique sorted order: rank by z.
-    // Only a column with equal z values (a perfect lattice) replays the selection sort, whose
-    // swaps decide the order of ties.
-    bool tie = false;
-    for (int p = lane; p < c; p += 64) {
-        const double zp = s_z[p];
-        int r = 0;
-        for (int q = 0; q < c; q++) {
-            const double zq = s_z[q];
-            r += zq < zp;
-            tie |= (zq == zp) && (q != p);
-        }
-        s_tmp[r] = s_idx[p];
-    }
-    if (!__any(tie)) {
-        __syncthreads();
-        for (int p = lane; p < c; p += 64)
-            atoms[s0 + p] = s_tmp[p];
-        if (lane == 0) {
-            int nc = (c + M - 1) / M;
-            if (N > M && (nc % 2)) nc++;
-            ncl[b] = nc;
-        }
-        return;
-    }
-    __syncthreads();
-    for (int a = 0; a < c; a++) {
-        double bz = INFINITY;
-        int bp    = INT_MAX;
-        for (int p = a + lane; p < c; p += 64) {
-            double zz = s_z[p];
-            if (zz < bz || (zz == bz && p < bp)) {
-                bz = zz;
-                bp = p;
-            }
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            double oz = __shfl_xor(bz, o, 64);
-            int op    = __shfl_xor(bp, o, 64);
-            if (oz < bz || (oz == bz && op < bp)) {
-                bz = oz;
-                bp = op;
-            }
-        }
-        if (lane == 0) {
-            int mp = (bp == INT_MAX) ? a : bp;
-            int ti = s_idx[a];
-            double tz = s_z[a];
-            s_idx[a]  = s_idx[mp];
-            s_z[a]    = s_z[mp];
-            s_idx[mp] = ti;
-            s_z[mp]   = tz;
-        }
-        __syncthreads();
-    }
-    for (int p = lane; p < c; p += 64)
-        atoms[s0 + p] = s_idx[p];
-    if (lane == 0) {
+    if (t == 0) {
+        s_tie  = 0;
         int nc = (c + M - 1) / M;
         if (N > M && (nc % 2)) nc++;
         ncl[b] = nc;
     }
+    for (int p = t; p < c; p += SORT_T) {
+        const int i = atoms[s0 + p];
+        s_idx[p]    = i;
+        s_z[p]      = ax[3 * i + 2];
+    }
+    __syncthreads();
+    bool tie = false;
+    for (int p = t; p < c; p += SORT_T) {
+        const double zp = s_z[p];
+        int r = 0, e = 0, q = 0;
+        for (; q + 2 <= c; q += 2) {
+            const double2 zq = *reinterpret_cast<const double2*>(&s_z[q]);
+            r += (zq.x < zp) + (zq.y < zp);
+            e += (zq.x == zp) + (zq.y == zp);
+        }
+        if (q < c) {
+            r += s_z[q] < zp;
+            e += s_z[q] == zp;
+        }
+        tie |= e > 1;
+        s_tmp[p] = r;
+    }
+    if (__any(tie) && lane == 0) s_tie = 1;
+    __syncthreads();
+    if (!s_tie) {
+        for (int p = t; p < c; p += SORT_T)
+            atoms[s0 + s_tmp[p]] = s_idx[p];
+        return;
+    }
+    // ties: index order first (indices are distinct), then the selection sort on one wavefront
+    for (int p = t; p < c; p += SORT_T) {
+        const int v = s_idx[p];
+        int r = 0, q = 0;
+        for (; q + 4 <= c; q += 4) {
+            const int4 w = *reinterpret_cast<const int4*>(&s_idx[q]);
+            r += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
+        }
+        for (; q < c; q++)
+            r += s_idx[q] < v;
+        s_tmp[r] = v;
+    }
+    __syncthreads();
+    for (int p = t; p < c; p += SORT_T) {
+        s_idx[p] = s_tmp[p];
+        s_z[p]   = ax[3 * s_tmp[p] + 2];
+    }
+    __syncthreads();
+    if (t < 64) {
+        for (int a = 0; a < c; a++) {
+            double bz = INFINITY;
+            int bp    = INT_MAX;
+            for (int p = a + lane; p < c; p += 64) {
+                double zz = s_z[p];
+                if (zz < bz || (zz == bz && p < bp)) {
+                    bz = zz;
+                    bp = p;
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                double oz = __shfl_xor(bz, o, 64);
+                int op    = __shfl_xor(bp, o, 64);
+                if (oz < bz || (oz == bz && op < bp)) {
+                    bz = oz;
+                    bp = op;
+                }
+            }
+            if (lane == 0) {
+                int mp    = (bp == INT_MAX) ? a : bp;
+                int ti    = s_idx[a];
+                double tz = s_z[a];
+                s_idx[a]  = s_idx[mp];
+                s_z[a]    = s_z[mp];
+                s_idx[mp] = ti;
+                s_z[mp]   = tz;
+            }
+            // one wavefront: its LDS operations complete in order
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    for (int p = t; p < c; p += SORT_T)
+        atoms[s0 + p] = s_idx[p];
 }
 
 __global__ void k_cp_fill_ci_bin(int mbins, const int* __restrict__ cstart, const int* __restrict__ ncl,
@@ -336,6 +379,8 @@ __global__ void k_cp_fill_ci_bin(int mbins, const int* __restrict__ cstart, cons
 }
 
 // buildClustersCPU (neighbor.c:1513-1611): cluster ci holds atoms cl*M .. cl*M+M-1 of its bin.
+// One lane per slot; a cluster's M lanes are adjacent (M divides 64) and reduce its bounding box
+// and atom count with cross-lane reads.
 __global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bin,
     const int* __restrict__ cstart, const int* __restrict__ abin_start,
     const int* __restrict__ abin_atoms, const double* __restrict__ ax,
@@ -343,28 +388,29 @@ __global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bi
     double* __restrict__ sv, int* __restrict__ st, int* __restrict__ ci_natoms,
     double* __restrict__ ci_bbox)
 {
-    int ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= nci) return;
-    int b  = ci_bin[ci];
-    int cl = ci - cstart[b];
-    int s0 = abin_start[b], c = abin_start[b + 1] - s0;
-    double bb[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
-    int na = 0;
-    for (int cii = 0; cii < M; cii++) {
-        int ac = cl * M + cii;
-        int s  = ci * M + cii;
-        if (ac < c) {
-            int i = abin_atoms[s0 + ac];
+    const int s    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int ci = s / M, cii = s % M;
+    const bool live = ci < nci;
+    bool has        = false;
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    if (live) {
+        const int b  = ci_bin[ci];
+        const int ac = (ci - cstart[b]) * M + cii;
+        const int s0 = abin_start[b], c = abin_start[b + 1] - s0;
+        has          = ac < c;
+        if (has) {
+            const int i = abin_atoms[s0 + ac];
+#pragma unroll
             for (int d = 0; d < 3; d++) {
-                double xv    = ax[3 * i + d];
-                sx[3 * s + d] = xv;
-                sv[3 * s + d] = av[3 * i + d];
-                if (bb[2 * d] > xv) bb[2 * d] = xv;
-                if (bb[2 * d + 1] < xv) bb[2 * d + 1] = xv;
+                const double xv = ax[3 * i + d];
+                sx[3 * s + d]   = xv;
+                sv[3 * s + d]   = av[3 * i + d];
+                lo[d] = hi[d] = xv;
             }
             st[s] = at[i];
-            na++;
         } else {
+#pragma unroll
             for (int d = 0; d < 3; d++) {
                 sx[3 * s + d] = pad_pos(s, d);
                 sv[3 * s + d] = 0.0;
@@ -372,9 +418,21 @@ __global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bi
             st[s] = 0;
         }
     }
-    ci_natoms[ci] = na;
-    for (int k = 0; k < 6; k++)
-        ci_bbox[6 * ci + k] = bb[k];
+    for (int o = 1; o < M; o <<= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    const unsigned long long bal = __ballot(has);
+    if (live && cii == 0) {
+        ci_natoms[ci] = __popcll((bal >> (lane & ~(M - 1))) & ((M >= 64) ? ~0ull : ((1ull << M) - 1)));
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ci_bbox[6 * ci + 2 * d]     = lo[d];
+            ci_bbox[6 * ci + 2 * d + 1] = hi[d];
+        }
+    }
 }
 
 // defineJClusters (neighbor.c:1807-1927), shapes M == N and N == 2M.
@@ -412,24 +470,31 @@ __global__ void k_cp_ghost_fill(int ncj, int N, const int* __restrict__ cj_natom
     const int* __restrict__ off, int* __restrict__ border, int* __restrict__ pbc,
     int* __restrict__ cj_natoms, int* __restrict__ st, int* __restrict__ status)
 {
-    int cj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cj >= ncj) return;
-    int na = cj_natoms_in[cj];
-    if (na <= 0) return;
-    unsigned m = ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut);
-    int g      = off[cj];
-    for (int c = 0; c < 26; c++) {
-        if (!(m & (1u << c))) continue;
-        border[g] = cj;
-        for (int d = 0; d < 3; d++)
-            pbc[3 * g + d] = c_side[c][d] == 1 ? 1 : (c_side[c][d] == 2 ? -1 : 0);
-        int cg        = ncj + g;
-        cj_natoms[cg] = na;
-        for (int cjj = 0; cjj < na; cjj++)
-            st[cg * N + cjj] = st[cj * N + cjj];
-        atomicAdd(&status[3], na);
-        g++;
+    const int cj = blockIdx.x * blockDim.x + threadIdx.x;
+    int nadd     = 0;  // ghost atoms this cluster creates
+    if (cj < ncj) {
+        const int na = cj_natoms_in[cj];
+        if (na > 0) {
+            unsigned m = ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut);
+            int g      = off[cj];
+            for (int c = 0; c < 26; c++) {
+                if (!(m & (1u << c))) continue;
+                border[g] = cj;
+                for (int d = 0; d < 3; d++)
+                    pbc[3 * g + d] = c_side[c][d] == 1 ? 1 : (c_side[c][d] == 2 ? -1 : 0);
+                int cg        = ncj + g;
+                cj_natoms[cg] = na;
+                for (int cjj = 0; cjj < na; cjj++)
+                    st[cg * N + cjj] = st[cj * N + cjj];
+                nadd += na;
+                g++;
+            }
+        }
     }
+    // one atomic per wavefront on the shared counter
+    for (int o = 32; o > 0; o >>= 1)
+        nadd += __shfl_xor(nadd, o, 64);
+    if ((threadIdx.x & 63) == 0 && nadd) atomicAdd(&status[3], nadd);
 }
 
 // updatePbcCPU with firstUpdate = 1: positions, padding and bounding boxes of ghost clusters.
@@ -577,6 +642,33 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
     }
 }
 
+// Coarse z index of every column: zidx[b * NZB + k] is the first position of column b (sorted by
+// bbminz) whose key is at least z0 + k dz.  The list build reads two entries per stencil column
+// instead of running two binary searches.
+constexpr int NZB = 256;
+// Bucket of z, clamped to [-4, NZB + 4] (NaN to -4) before the conversion to int.
+__device__ inline int zbucket(double z, double z0, double inv_dz)
+{
+    const double t = floor((z - z0) * inv_dz);
+    if (!(t >= -4.0)) return -4;
+    return t > (double)(NZB + 4) ? NZB + 4 : (int)t;
+}
+__global__ void k_cp_zindex(int mbins, const int* __restrict__ jstart,
+    const double* __restrict__ jkey, double z0, double dz, int* __restrict__ zidx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mbins * NZB) return;
+    const int b = t / NZB, k = t % NZB;
+    const double zb = z0 + k * dz;
+    int lo = jstart[b], hi = jstart[b + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (jkey[mid] < zb) lo = mid + 1;
+        else hi = mid;
+    }
+    zidx[t] = lo;
+}
+
 // ---------------------------------------------------------------- buildNeighborCPU (neighbor.c:282-766)
 // Largest z-extent of any j-cluster (local or ghost): bounds bbmaxz from bbminz in the column
 // search below.  Positive doubles order like their bit patterns.
@@ -598,24 +690,11 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
     }
 }
 
-// First position m in [lo, hi) of a bbminz-sorted column whose key is >= v (strict: > v).
-template <bool STRICT>
-__device__ __forceinline__ int col_search(const double* __restrict__ jkey, int lo, int hi,
-    double v)
-{
-    while (lo < hi) {
-        int mid  = (lo + hi) >> 1;
-        double k = jkey[mid];
-        if (STRICT ? (k <= v) : (k < v)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
 // One wavefront per i-cluster.  Each stencil column is sorted by bbminz, so the j-clusters
 // whose z-gap to the i-cluster alone exceeds the list cutoff form a prefix and a suffix of the
-// column; two binary searches cut them off (a j-cluster's bbmaxz is at most its bbminz plus the
-// largest cluster height).  Dropped candidates are exactly ones the bounding-box test would
+// column; the column's coarse z index cuts most of them off (a j-cluster's bbmaxz is at most its
+// bbminz plus the largest cluster height).  Dropped candidates are exactly ones the bounding-box test would
 // reject, so the list is the reference's, in the reference's order (stencil columns in order,
 // clusters in column order).  The remaining candidates of up to 64 columns are flattened and
 // tested 64 at a time; ballots give every accepted j-cluster its place in the list.
@@ -624,7 +703,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ cj_bbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
-    const double* __restrict__ jkey, const double* __restrict__ sx,
+    const int* __restrict__ zidx, double z0, double inv_dz, const double* __restrict__ sx,
     double cutneighsq, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
     int maxneighs, int* __restrict__ nb, int* __restrict__ numneigh, int* __restrict__ nmasked,
     int* __restrict__ status)
@@ -635,7 +714,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
         s_st[k] = stencil[k];
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ci = blockIdx.x * (BLOCK / 64) + w;
+    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + w);
     if (ci >= nci) return;
     const int ci_cj0 = (M == N) ? ci : ci >> 1;
     const int ibin   = ci_bin[ci];
@@ -643,6 +722,12 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
 #pragma unroll
     for (int k = 0; k < 6; k++)
         ib[k] = ci_bbox[6 * ci + k];
+    double xi[M][3];  // the i-cluster's slots (empty ones sit at far sentinel positions)
+#pragma unroll
+    for (int cii = 0; cii < M; cii++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            xi[cii][d] = sx[3 * (ci * M + cii) + d];
     // search window in bbminz; the margin only widens it
     const double R    = sqrt(cutneighsq) * (1.0 + 1e-12) + 1e-9;
     const double hmax = __longlong_as_double((long long)*hmax_bits);
@@ -659,10 +744,13 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
             if (jb < 0 || jb >= mbins) {
                 atomicOr(status, ST_BIN_RANGE);
             } else {
-                const int beg = jstart[jb], end = jstart[jb + 1];
-                lo           = col_search<false>(jkey, beg, end, zlo);
-                const int hi = col_search<true>(jkey, lo, end, zhi);
-                cnt          = hi - lo;
+                // bucket boundaries one below zlo and one above zhi (a rounding of the bucket
+                // index can only move a boundary by one): the window only widens
+                const int kl = zbucket(zlo, z0, inv_dz) - 1;
+                const int kh = zbucket(zhi, z0, inv_dz) + 2;
+                lo           = kl < 0 ? jstart[jb] : (kl >= NZB ? jstart[jb + 1] : zidx[jb * NZB + kl]);
+                const int hi = kh >= NZB ? jstart[jb + 1] : (kh < 0 ? jstart[jb] : zidx[jb * NZB + kh]);
+                cnt          = hi > lo ? hi - lo : 0;
             }
         }
         // exclusive scan of the per-column counts
@@ -701,19 +789,25 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
                     }
                     if (d2 < cutneighsq) {
                         pass = d2 < rbb_sq;
-                        for (int cii = 0; cii < M && !pass; cii++) {
-                            const double* xi = &sx[3 * (ci * M + cii)];
-                            const double xa = xi[0], ya = xi[1], za = xi[2];
-                            for (int cjj = 0; cjj < N; cjj++) {
-                                const double* xj = &sx[3 * (cj * N + cjj)];
-                                double dx = xa - xj[0], dy = ya - xj[1], dz = za - xj[2];
-                                double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
-                                                      : dx * dx + dy * dy + dz * dz;
-                                if (rsq < cutneighsq) {
-                                    pass = true;
-                                    break;
+                        if (!pass) {
+                            // any atom pair inside the cutoff: all N positions load at once
+                            double xj[N][3];
+#pragma unroll
+                            for (int cjj = 0; cjj < N; cjj++)
+#pragma unroll
+                                for (int d = 0; d < 3; d++)
+                                    xj[cjj][d] = sx[3 * (cj * N + cjj) + d];
+#pragma unroll
+                            for (int cii = 0; cii < M; cii++)
+#pragma unroll
+                                for (int cjj = 0; cjj < N; cjj++) {
+                                    double dx = xi[cii][0] - xj[cjj][0];
+                                    double dy = xi[cii][1] - xj[cjj][1];
+                                    double dz = xi[cii][2] - xj[cjj][2];
+                                    double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                                          : dx * dx + dy * dy + dz * dz;
+                                    pass = pass || rsq < cutneighsq;
                                 }
-                            }
                         }
                     }
                     masked = pass && imask_of(rule, ci, cj) != IMASK_ALL;
@@ -750,6 +844,83 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
             }
         }
     }
+}
+
+// ---------------------------------------------------------------- pruneNeighborCPU (neighbor.c:1099-1264)
+// One wavefront per i-cluster.  Whether a listed j-cluster still has an atom pair inside the
+// list cutoff is a property of the entry alone, so the lanes test all entries first (64 at a
+// time, all M x N pairs each; empty slots sit at far sentinel positions).  The reference then
+// walks the list once: an entry out of range is overwritten by the current last entry, which is
+// tested in its place, and every removal at a position inside the masked prefix shortens that
+// prefix.  One lane replays exactly that walk on the entries staged in LDS.
+constexpr int PRUNE_CAP = 512;  // list entries per i-cluster staged in LDS
+template <int M, int N>
+__global__ void __launch_bounds__(BLOCK) k_cp_prune(int nci, int fma_dist,
+    const double* __restrict__ sx, double cutneighsq, int maxneighs, int* __restrict__ nb,
+    int* __restrict__ numneigh, int* __restrict__ nmasked, int* __restrict__ status)
+{
+    __shared__ int s_e[BLOCK / 64][PRUNE_CAP];
+    __shared__ unsigned char s_in[BLOCK / 64][PRUNE_CAP];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + w);
+    if (ci >= nci) return;
+    const int nn = numneigh[ci];
+    if (nn > PRUNE_CAP) {
+        if (lane == 0) atomicOr(status, ST_NB_OVERFLOW);
+        return;
+    }
+    double xi[M][3];
+#pragma unroll
+    for (int cii = 0; cii < M; cii++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            xi[cii][d] = sx[3 * (ci * M + cii) + d];
+    int* lst = nb + (size_t)ci * maxneighs;
+    for (int k = lane; k < nn; k += 64) {
+        const int cj = lst[k];
+        double xj[N][3];
+#pragma unroll
+        for (int cjj = 0; cjj < N; cjj++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                xj[cjj][d] = sx[3 * (cj * N + cjj) + d];
+        bool in = false;
+#pragma unroll
+        for (int cii = 0; cii < M; cii++)
+#pragma unroll
+            for (int cjj = 0; cjj < N; cjj++) {
+                double dx = xi[cii][0] - xj[cjj][0];
+                double dy = xi[cii][1] - xj[cjj][1];
+                double dz = xi[cii][2] - xj[cjj][2];
+                double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                      : dx * dx + dy * dy + dz * dz;
+                in = in || rsq < cutneighsq;
+            }
+        s_e[w][k]  = cj;
+        s_in[w][k] = in ? 1 : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __shared__ int s_n[BLOCK / 64];
+    if (lane == 0) {
+        int n = nn, nm = nmasked[ci], k = 0;
+        while (k < n) {
+            if (s_in[w][k]) {
+                k++;
+            } else {
+                n--;
+                if (k < nm) nm--;
+                s_e[w][k]  = s_e[w][n];
+                s_in[w][k] = s_in[w][n];
+            }
+        }
+        numneigh[ci] = n;
+        nmasked[ci]  = nm;
+        s_n[w]       = n;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int n = s_n[w];
+    for (int k = lane; k < n; k += 64)
+        lst[k] = s_e[w][k];
 }
 
 // ---------------------------------------------------------------- FP64 force, reference arithmetic
@@ -867,7 +1038,7 @@ struct CPAcc {
 
     __device__ __forceinline__ void pair(const JPos& J, double xi, double yi, double zi, int si,
         int ci_cj0, int ncj, int a, int ti, int ntypes, const int* __restrict__ st,
-        const float (*s_tab)[MAXT2], const double* s_cut, float eps48u, float eps4u,
+        const float (*s_tab)[MAXT2], const double* s_cut, float c12u, float nc6u, float eps4u,
         float sig6u, float clo_u, float chi_u, double cut2u, float* __restrict__ f)
     {
         const double dxd = xi - J.x;
@@ -875,30 +1046,38 @@ struct CPAcc {
         const double dzd = zi - J.z;
         const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
         const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        float eps48 = eps48u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
+        float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
         double cut2 = cut2u;
         if (!UNIFORM) {
             const int tij = ti * ntypes + st[J.sj];
-            eps48         = s_tab[0][tij];
-            eps4          = s_tab[1][tij];
-            sig6          = s_tab[2][tij];
+            c12           = s_tab[0][tij];
+            nc6           = -s_tab[1][tij];
+            eps4          = s_tab[2][tij];
+            sig6          = s_tab[3][tij];
             cut2          = s_cut[tij];
             clo           = (float)cut2 * (1.0f - CUT_BAND);
             chi           = (float)cut2 * (1.0f + CUT_BAND);
         }
-        bool in         = rsq < clo;
-        const bool near = !in && rsq < chi;
-        if (__builtin_expect(__any(near), 0)) {
+        bool in          = rsq < clo;
+        const bool below = rsq < chi;
+        // lanes in the band: below and not in, from the two lane masks
+        if (__builtin_expect((__ballot(below) & ~__ballot(in)) != 0, 0)) {
             const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
-            if (near) in = rsqd < cut2;
+            if (below && !in) in = rsqd < cut2;
         }
         if (HALF) in = in && !(J.cj == ci_cj0 && !(si < J.sj));
         else in = in && rsq > 0.0f;
-        float r2i       = __builtin_amdgcn_rcpf(rsq);
-        r2i             = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
-        const float sr6 = r2i * r2i * r2i * sig6;
-        float fpair     = eps48 * sr6 * (sr6 - 0.5f) * r2i;
+        float r2i = __builtin_amdgcn_rcpf(rsq);
+        r2i       = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
+        const float r6i = r2i * r2i * r2i;
+        float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
         fpair           = in ? fpair : 0.0f;
+        if (EV) {
+            const float sr6 = r6i * sig6;
+            const float wgt = (HALF && J.cj < ncj) ? 1.0f : 0.5f;
+            e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
+            w += wgt * rsq * fpair;
+        }
         if (COMP) {
             kahan_add(fx, cx, dx * fpair);
             kahan_add(fy, cy, dy * fpair);
@@ -907,11 +1086,6 @@ struct CPAcc {
             fx = fmaf(dx, fpair, fx);
             fy = fmaf(dy, fpair, fy);
             fz = fmaf(dz, fpair, fz);
-        }
-        if (EV) {
-            const float wgt = (HALF && J.cj < ncj) ? 1.0f : 0.5f;
-            e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
-            w += wgt * rsq * fpair;
         }
         if (HALF) {
             // j-force: reduce over the four i-lanes that share this j-atom, one atomic per atom
@@ -939,13 +1113,14 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
     double* __restrict__ ev)
 {
     constexpr int CPI = 16 / N;  // j-clusters per iteration
-    __shared__ float s_tab[3][MAXT2];
+    __shared__ float s_tab[4][MAXT2];
     __shared__ double s_cut[MAXT2];
     if (!UNIFORM) {
         for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
-            s_tab[0][k] = tab.eps48[k];
-            s_tab[1][k] = tab.eps4[k];
-            s_tab[2][k] = tab.sig6[k];
+            s_tab[0][k] = tab.c12[k];
+            s_tab[1][k] = tab.c6[k];
+            s_tab[2][k] = tab.eps4[k];
+            s_tab[3][k] = tab.sig6[k];
             s_cut[k]    = tab.cut2d[k];
         }
         __syncthreads();
@@ -961,6 +1136,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
     const int lane = threadIdx.x & 63;
     const int a = lane & 3, s = lane >> 2;
     const int g = s / N, jj = s % N;
+    const int g4 = g << 2;
+    const unsigned jj24 = 24u * (unsigned)jj;
     const int si     = ci * 4 + a;
     const bool ilive = a < ci_natoms[ci];
     const double xi = sx[3 * si], yi = sx[3 * si + 1], zi = sx[3 * si + 2];
@@ -968,7 +1145,10 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
     const int ti     = UNIFORM ? 0 : st[si];
     const int nn     = __builtin_amdgcn_readfirstlane(numneigh[ci]);
     const int* lst   = nb + (size_t)ci * maxneighs;
-    const float eps48u = tab.eps48[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
+    const float c12u = tab.c12[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
+    // -c6 in a vector register once (a VALU op reads at most one scalar operand)
+    float nc6u = -tab.c6[0];
+    asm volatile("" : "+v"(nc6u));
     const double cut2u = tab.cut2d[0];
     const float clo_u  = (float)cut2u * (1.0f - CUT_BAND), chi_u = (float)cut2u * (1.0f + CUT_BAND);
     CPAcc<N, HALF, EV, UNIFORM, COMP> acc;
@@ -979,8 +1159,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
     // the loop is unconditional (a fetch past the chunk reads valid entries and is discarded),
     // so the compiler's wait before a use waits only for that use's own loads.
     auto use = [&](const JPos& J) {
-        acc.pair(J, xi, yi, zi, si, ci_cj0, ncj, a, ti, ntypes, st, s_tab, s_cut, eps48u, eps4u,
-            sig6u, clo_u, chi_u, cut2u, f);
+        acc.pair(J, xi, yi, zi, si, ci_cj0, ncj, a, ti, ntypes, st, s_tab, s_cut, c12u, nc6u,
+            eps4u, sig6u, clo_u, chi_u, cut2u, f);
     };
     for (int c0 = 0; c0 < nn; c0 += 128) {
         const int e0   = c0 + lane < nn ? lst[c0 + lane] : dummy_cj;
@@ -988,11 +1168,16 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
         const int cend = min(nn, c0 + 128);
         auto fetch = [&](int kb, JPos& J) {
             const int src = (kb & 64) ? e1 : e0;  // kb is wave-uniform
-            J.cj = __shfl(src, (kb + g) & 63, 64);
+            // kb is a multiple of CPI and g < CPI, so (kb & 63) + g is the source lane
+            J.cj = __builtin_amdgcn_ds_bpermute(((kb & 63) << 2) + g4, src);
             J.sj = J.cj * N + jj;
-            J.x  = sx[3 * J.sj];
-            J.y  = sx[3 * J.sj + 1];
-            J.z  = sx[3 * J.sj + 2];
+            // 32-bit byte offset from the base, one 24-bit multiply-add (the host keeps every
+            // j-cluster index below 2^24 and every offset below 2^32)
+            const double* p = reinterpret_cast<const double*>(
+                reinterpret_cast<const char*>(sx) + (__umul24(J.cj, 24u * N) + jj24));
+            J.x = p[0];
+            J.y = p[1];
+            J.z = p[2];
         };
         JPos A, B;
         fetch(c0, A);
@@ -1047,6 +1232,202 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
             atomicAdd(&ev[1], wd);
         }
     }
+}
+
+// Full lists, j-atom lanes.  One wavefront works on four i-clusters: the 16 lanes of group g
+// take i-cluster 4w+g, hold its four i-atom positions, and each lane takes one j-atom per
+// iteration (16/N j-clusters of the group's list per iteration) and computes that j-atom's pairs
+// with all four i-atoms.  Every j position is loaded by one lane only and serves four pairs, and
+// each lane keeps the forces of its group's four i-atoms; the 16 lanes of a group reduce them at
+// the end.  The four lists of a wavefront run to the longest; shorter ones read the dummy
+// j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and arithmetic
+// are those of k_cp_force_f32.
+template <int N, bool EV, bool UNIFORM, bool COMP>
+__global__ void __launch_bounds__(BLOCK) k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
+    int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
+    const int* __restrict__ nb, const int* __restrict__ numneigh,
+    const int* __restrict__ ci_natoms, int maxneighs, float* __restrict__ f,
+    double* __restrict__ ev, double* __restrict__ v, double dtforce)
+{
+    constexpr int CPG = 16 / N;  // j-clusters per group per iteration
+    __shared__ float s_tab[4][MAXT2];
+    __shared__ double s_cut[MAXT2];
+    if (!UNIFORM) {
+        for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
+            s_tab[0][k] = tab.c12[k];
+            s_tab[1][k] = tab.c6[k];
+            s_tab[2][k] = tab.eps4[k];
+            s_tab[3][k] = tab.sig6[k];
+            s_cut[k]    = tab.cut2d[k];
+        }
+        __syncthreads();
+    }
+    // XCD-aware block order, as in k_cp_force_f32 (the grid is a multiple of 8 blocks)
+    const int per  = gridDim.x >> 3;
+    const int lb   = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int wv   = __builtin_amdgcn_readfirstlane(lb * (BLOCK / 64) + (threadIdx.x >> 6));
+    if (wv * 4 >= nci) return;
+    const int lane = threadIdx.x & 63, grp = lane >> 4, sub = lane & 15;
+    const int jc = sub / N, jj = sub % N;
+    const unsigned jj24 = 24u * (unsigned)jj;
+    const int ci    = wv * 4 + grp;
+    const bool live = ci < nci;
+    double xi[4][3];
+    int ti[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            xi[a][d] = live ? sx[3 * (ci * 4 + a) + d] : 0.0;
+        ti[a] = (UNIFORM || !live) ? 0 : st[ci * 4 + a];
+    }
+    const int nn = live ? numneigh[ci] : 0;
+    int nmax     = max(nn, __shfl_xor(nn, 16, 64));
+    nmax         = max(nmax, __shfl_xor(nmax, 32, 64));
+    nmax         = __builtin_amdgcn_readfirstlane(nmax);
+    const int* lst = nb + (size_t)(live ? ci : 0) * maxneighs;
+    const float c12u = tab.c12[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
+    float nc6u = -tab.c6[0];
+    asm volatile("" : "+v"(nc6u));
+    const double cut2u = tab.cut2d[0];
+    const float clo_u  = (float)cut2u * (1.0f - CUT_BAND), chi_u = (float)cut2u * (1.0f + CUT_BAND);
+    float fx[4] = {}, fy[4] = {}, fz[4] = {}, cx[4] = {}, cy[4] = {}, cz[4] = {};
+    float e = 0.f, w = 0.f;
+    struct JL {
+        double x, y, z;
+        int sj;
+    };
+    auto use = [&](const JL& J) {
+        const int tj = UNIFORM ? 0 : st[J.sj];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const double dxd = xi[a][0] - J.x;
+            const double dyd = xi[a][1] - J.y;
+            const double dzd = xi[a][2] - J.z;
+            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
+            double cut2 = cut2u;
+            if (!UNIFORM) {
+                const int tij = ti[a] * ntypes + tj;
+                c12           = s_tab[0][tij];
+                nc6           = -s_tab[1][tij];
+                eps4          = s_tab[2][tij];
+                sig6          = s_tab[3][tij];
+                cut2          = s_cut[tij];
+                clo           = (float)cut2 * (1.0f - CUT_BAND);
+                chi           = (float)cut2 * (1.0f + CUT_BAND);
+            }
+            bool in          = rsq < clo;
+            const bool below = rsq < chi;
+            if (__builtin_expect((__ballot(below) & ~__ballot(in)) != 0, 0)) {
+                const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+                if (below && !in) in = rsqd < cut2;
+            }
+            in        = in && rsq > 0.0f;  // the self pair is the only one at distance 0
+            float r2i = __builtin_amdgcn_rcpf(rsq);
+            r2i       = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);
+            const float r6i = r2i * r2i * r2i;
+            float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
+            fpair           = in ? fpair : 0.0f;
+            if (COMP) {
+                kahan_add(fx[a], cx[a], dx * fpair);
+                kahan_add(fy[a], cy[a], dy * fpair);
+                kahan_add(fz[a], cz[a], dz * fpair);
+            } else {
+                fx[a] = fmaf(dx, fpair, fx[a]);
+                fy[a] = fmaf(dy, fpair, fy[a]);
+                fz[a] = fmaf(dz, fpair, fz[a]);
+            }
+            if (EV) {
+                const float sr6 = r6i * sig6;
+                e += in ? 0.5f * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
+                w += 0.5f * rsq * fpair;
+            }
+        }
+    };
+    // Entries of the group's list are held 32 at a time in two registers per lane (16 each) and
+    // handed out with cross-lane reads inside the group; entries past a list's end read as the
+    // dummy j-cluster.  Positions of the next iteration load while the current ones are used,
+    // and every load in the loop is unconditional (see k_cp_force_f32).
+    for (int c0 = 0; c0 < nmax; c0 += 32) {
+        const int e0   = c0 + sub < nn ? lst[c0 + sub] : dummy_cj;
+        const int e1   = c0 + 16 + sub < nn ? lst[c0 + 16 + sub] : dummy_cj;
+        const int cend = min(nmax, c0 + 32);
+        auto fetch = [&](int kb, JL& J) {
+            const int src = (kb & 16) ? e1 : e0;  // kb is wave-uniform, a multiple of CPG
+            const int cj  = __builtin_amdgcn_ds_bpermute(((grp << 4) + (kb & 15) + jc) << 2, src);
+            J.sj          = cj * N + jj;
+            const double* p = reinterpret_cast<const double*>(
+                reinterpret_cast<const char*>(sx) + (__umul24(cj, 24u * N) + jj24));
+            J.x = p[0];
+            J.y = p[1];
+            J.z = p[2];
+        };
+        JL A, B;
+        fetch(c0, A);
+        for (int k0 = c0; k0 < cend;) {
+            fetch(k0 + CPG, B);
+            use(A);
+            k0 += CPG;
+            if (k0 >= cend) break;
+            fetch(k0 + CPG, A);
+            use(B);
+            k0 += CPG;
+        }
+    }
+    // reduce over the 16 lanes of each group (lane bits 0..3)
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float ox = __shfl_xor(fx[a], o, 64), oy = __shfl_xor(fy[a], o, 64),
+                        oz = __shfl_xor(fz[a], o, 64);
+            if (COMP) {
+                const float ocx = __shfl_xor(cx[a], o, 64), ocy = __shfl_xor(cy[a], o, 64),
+                            ocz = __shfl_xor(cz[a], o, 64);
+                two_sum_merge(fx[a], cx[a], ox, ocx);
+                two_sum_merge(fy[a], cy[a], oy, ocy);
+                two_sum_merge(fz[a], cz[a], oz, ocz);
+            } else {
+                fx[a] += ox;
+                fy[a] += oy;
+                fz[a] += oz;
+            }
+        }
+    }
+    if (live && sub < 4 && sub < ci_natoms[ci]) {
+        float rx = 0.f, ry = 0.f, rz = 0.f;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            if (a == sub) {
+                rx = COMP ? fx[a] + cx[a] : fx[a];
+                ry = COMP ? fy[a] + cy[a] : fy[a];
+                rz = COMP ? fz[a] + cz[a] : fz[a];
+            }
+        const int si  = ci * 4 + sub;
+        f[3 * si]     = rx;
+        f[3 * si + 1] = ry;
+        f[3 * si + 2] = rz;
+        if (v) {
+            // finalIntegrate (integrate.c:61-86) of this atom, fused: its force is complete
+            v[3 * si]     = v[3 * si] + dtforce * (double)rx;
+            v[3 * si + 1] = v[3 * si + 1] + dtforce * (double)ry;
+            v[3 * si + 2] = v[3 * si + 2] + dtforce * (double)rz;
+        }
+    }
+    if (EV) {
+        double ed = e, wd = w;
+        for (int o = 32; o > 0; o >>= 1) {
+            ed += __shfl_xor(ed, o, 64);
+            wd += __shfl_xor(wd, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&ev[0], ed);
+            atomicAdd(&ev[1], wd);
+        }
+    }
+    (void)ncj;
 }
 
 // ---------------------------------------------------------------- integrate (integrate.c:29-86)
@@ -1185,7 +1566,8 @@ void initDeviceClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
         C.t64.eps[k]    = atom->epsilon[k];
         C.t64.sig6[k]   = atom->sigma6[k];
         C.t64.cutf[k]   = atom->cutforcesq[k];
-        C.t32.eps48[k]  = (float)(48.0 * atom->epsilon[k]);
+        C.t32.c12[k]    = (float)(48.0 * atom->epsilon[k] * atom->sigma6[k] * atom->sigma6[k]);
+        C.t32.c6[k]     = (float)(24.0 * atom->epsilon[k] * atom->sigma6[k]);
         C.t32.eps4[k]   = (float)(4.0 * atom->epsilon[k]);
         C.t32.sig6[k]   = (float)atom->sigma6[k];
         C.t32.cut2[k]   = (float)atom->cutforcesq[k];
@@ -1206,8 +1588,8 @@ void freeDeviceClusterGFX950(void)
         C.d_abin_start, C.d_abin_atoms, C.d_ncl, C.d_cstart, C.d_ci_natoms, C.d_ci_bin,
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
-        C.d_jlist, C.d_jkey, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb, C.d_numneigh,
-        C.d_nmasked, C.d_red, C.d_ev, C.d_status };
+        C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
+        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status };
     for (void* p : ptrs)
         dfree(p);
     if (C.h_status) (void)hipHostFree(C.h_status);
@@ -1383,7 +1765,7 @@ void buildClustersGFX950(Atom* atom)
     fill_i32(C.d_abin_cnt, 0, (size_t)mb);
     k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
         C.d_abin_cnt, C.d_abin_atoms);
-    k_cp_bin_sort<<<mb, 64, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
+    k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
         C.d_status);
     exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb);
     MDG_LAUNCH_CHECK("cp_binning");
@@ -1405,7 +1787,7 @@ void buildClustersGFX950(Atom* atom)
     ensure_slots((size_t)nci * C.M + C.N, 0);
     if (failed()) return;
     k_cp_fill_ci_bin<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.d_cstart, C.d_ncl, C.d_ci_bin);
-    k_cp_fill_clusters<<<blocks_for(nci, BLOCK), BLOCK, 0, s>>>(nci, C.M, C.d_ci_bin, C.d_cstart,
+    k_cp_fill_clusters<<<blocks_for(nci * C.M, BLOCK), BLOCK, 0, s>>>(nci, C.M, C.d_ci_bin, C.d_cstart,
         C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx, C.d_sv, C.d_st,
         C.d_ci_natoms, C.d_ci_bbox);
     MDG_LAUNCH_CHECK("cp_fill_clusters");
@@ -1511,6 +1893,18 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
         C.d_gcur, C.d_gbin_list);
     k_cp_jbin_merge<<<mb, 64, 0, s>>>(C.M, C.N, C.ncj, C.d_ncl, C.d_cstart, C.d_gbin_start,
         C.d_gbin_list, C.d_cj_bbox, C.d_jstart, C.d_jlist, C.d_jkey, C.d_status);
+    // z boundaries spanning the ghost layers below and above the box; keys outside the span
+    // only make the windows wider
+    grow_n(&C.d_zidx, &C.cap_zidx, (size_t)mb * NZB, 0);
+    if (failed()) return;
+    {
+        const double zlo_all = -(C.cutneigh + 4.0), zhi_all = param->zprd + C.cutneigh + 4.0;
+        const double dz      = (zhi_all - zlo_all) / (NZB - 1);
+        C.z0                 = zlo_all;
+        C.inv_dz             = 1.0 / dz;
+        k_cp_zindex<<<blocks_for((long long)mb * NZB, BLOCK), BLOCK, 0, s>>>(mb, C.d_jstart,
+            C.d_jkey, C.z0, dz, C.d_zidx);
+    }
     MDG_LAUNCH_CHECK("cp_bin_j");
 }
 
@@ -1538,7 +1932,8 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
         C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_cj_bbox, C.d_jstart,      \
-        C.d_jlist, C.d_jkey, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs, C.d_nb,         \
+        C.d_jlist, C.d_zidx, C.z0, C.inv_dz, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs,  \
+        C.d_nb,                                                                                  \
         C.d_numneigh,                                                                            \
         C.d_nmasked, C.d_status)
         if (C.M == 4 && C.N == 4) MDG_NB(4, 4);
@@ -1563,20 +1958,74 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     set_error(E_CAPACITY, "cluster-pair list did not converge");
 }
 
+void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
+{
+    (void)param;
+    (void)atom;
+    if (!ok()) return;
+    TimedRegion tr(F_NEIGHBOR);
+    hipStream_t s = stream();
+    const int nci = C.nci;
+    if (nci <= 0) return;
+    fill_i32(C.d_status, 0, 2);
+    unsigned grid = blocks_for(nci, BLOCK / 64);
+#define MDG_PRUNE(MM, NN)                                                                        \
+    k_cp_prune<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.fma_dist, C.d_sx, C.cutneighsq, C.maxneighs,  \
+        C.d_nb, C.d_numneigh, C.d_nmasked, C.d_status)
+    if (C.M == 4 && C.N == 4) MDG_PRUNE(4, 4);
+    else if (C.M == 4 && C.N == 8) MDG_PRUNE(4, 8);
+    else MDG_PRUNE(1, 2);
+#undef MDG_PRUNE
+    MDG_LAUNCH_CHECK("cp_prune");
+    read_status();
+    if (failed()) return;
+    if (C.h_status[0] & ST_NB_OVERFLOW)
+        set_error(E_CAPACITY, "cluster-pair prune: a list longer than %d entries", PRUNE_CAP);
+    neighbor->maxneighs = C.maxneighs;
+}
+
+// The force call; with fuse_final the FP32 full-list kernel also applies finalIntegrate to each
+// atom whose total force it has just written, and returns true (false: not fused, the caller
+// integrates).
+static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final, double* secs);
+
 double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
 {
     (void)atom;
     (void)stats;
-    if (!ok()) return 0.0;
+    double secs = 0.0;
+    force_cluster(param, neighbor, false, &secs);
+    return secs;
+}
+
+double computeForceFinalClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats)
+{
+    (void)stats;
+    double secs = 0.0;
+    if (!force_cluster(param, neighbor, true, &secs)) finalIntegrateClusterGFX950(param, atom);
+    return secs;
+}
+
+static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final, double* secs)
+{
+    if (!ok()) return false;
     const double t0 = wtime();
+    bool fused      = false;
     hipStream_t s   = stream();
     const int nci = C.nci, half = neighbor->half_neigh;
     const size_t nloc = (size_t)nci * C.M;
     if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
     TimedRegion tr(F_FORCE_CP);
+    if (C.precision == 0 &&
+        ((size_t)C.ncj + C.nghost + 1 >= (1u << 24) || nslots_all() * 24 >= (1ull << 32))) {
+        set_error(E_ARG, "FP32 cluster kernel: %zu slots exceed its 32-bit addressing",
+            nslots_all());
+        return false;
+    }
     if (C.precision == 1) {
         grow_n(&C.d_sf64, &C.cap_sf64, 3 * nslots_all(), 0);
-        if (failed()) return 0.0;
+        if (failed()) return false;
         if (half) fill_f64(C.d_sf64, 0.0, 3 * nslots_all());
         unsigned grid = blocks_for((long long)nloc, BLOCK);
         const int uni = C.uniform ? 1 : 0;
@@ -1601,13 +2050,24 @@ double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neigh
         C.forces_are_f32 = false;
     } else {
         grow_n(&C.d_sf32, &C.cap_sf32, 3 * nslots_all(), 0);
-        if (failed()) return 0.0;
+        if (failed()) return false;
         if (half) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
         unsigned grid = (blocks_for(nci, BLOCK / 64) + 7) & ~7u;  // multiple of 8, see kernel
+        // full lists: four i-clusters per wavefront (and the fused final integrate)
+        fused          = fuse_final && !half;
+        double* fuse_v = fused ? C.d_sv : nullptr;
+        unsigned grid4 = (blocks_for((nci + 3) / 4, BLOCK / 64) + 7) & ~7u;
 #define MDG_F32(NN, H, E, U, CO)                                                                 \
-    k_cp_force_f32<NN, H, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj, C.ntypes,      \
-        C.t32, C.d_sx,                                                                            \
-        C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms, C.maxneighs, C.d_sf32, C.d_ev)
+    do {                                                                                         \
+        if (H)                                                                                   \
+            k_cp_force_f32<NN, true, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
+                C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
+                C.maxneighs, C.d_sf32, C.d_ev);                                                   \
+        else                                                                                     \
+            k_cp_force_f32_full<NN, E, U, CO><<<grid4, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
+                C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
+                C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce);                          \
+    } while (0)
 #define MDG_F32_U(NN, H, E)                                                                      \
     do {                                                                                         \
         if (C.uniform) {                                                                         \
@@ -1636,8 +2096,8 @@ double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neigh
         C.forces_are_f32 = true;
     }
     MDG_LAUNCH_CHECK("cp_force");
-    (void)param;
-    return wtime() - t0;
+    *secs = wtime() - t0;
+    return fused;
 }
 
 void initialIntegrateClusterGFX950(Parameter* param, Atom* atom)
@@ -1712,6 +2172,20 @@ void clusterInfoGFX950(int* info)
     int v[11] = { C.nci, C.ncj, C.nghost, C.dummy_cj, C.nghost_atoms, C.g.mbins, C.nstencil,
         C.maxneighs, C.M, C.N, C.natoms };
     memcpy(info, v, sizeof(v));
+}
+
+long long uploadClusterArrayGFX950(const char* name, const void* in, long long n)
+{
+    if (!ok()) return -3;
+    void* dst       = nullptr;
+    long long avail = 0;
+    if (!strcmp(name, "slot_x")) { dst = C.d_sx; avail = 3LL * (long long)nslots_all(); }
+    else if (!strcmp(name, "slot_v")) { dst = C.d_sv; avail = 3LL * C.nci * C.M; }
+    if (!dst) return -1;
+    if (n > avail || n < 0) return -2;
+    MDG_HIP(hipMemcpyAsync(dst, in, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream()));
+    MDG_HIP(hipStreamSynchronize(stream()));
+    return failed() ? -3 : n;
 }
 
 long long downloadClusterArrayGFX950(const char* name, void* out, long long cap)

@@ -63,6 +63,7 @@ struct Pending {
     int fam;
 };
 static bool g_timing = false;
+static unsigned g_timed_mask = ~0u;  // families that record events while timing is on
 static double g_ms[F_NFAMILY];
 static long long g_launch[F_NFAMILY];
 static std::vector<Pending> g_pending;
@@ -98,7 +99,7 @@ static void harvest()
 
 TimedRegion::TimedRegion(Family f) : fam(f), slot(-1)
 {
-    if (!g_timing || failed()) return;
+    if (!g_timing || !(g_timed_mask & (1u << f)) || failed()) return;
     Pending p;
     p.a   = take_event();
     p.b   = take_event();
@@ -294,6 +295,20 @@ void mdgfx950EnableTiming(int on)
     }
     if (!on) mdg::harvest();
     mdg::g_timing = on != 0;
+}
+
+int mdgfx950TimeOnly(const char* family)
+{
+    if (!family) {
+        mdg::g_timed_mask = ~0u;
+        return 0;
+    }
+    for (int f = 0; f < mdg::F_NFAMILY; f++)
+        if (strcmp(family, mdg::g_famname[f]) == 0) {
+            mdg::g_timed_mask = 1u << f;
+            return 0;
+        }
+    return -1;
 }
 
 int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches)

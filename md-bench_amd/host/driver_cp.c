@@ -74,7 +74,7 @@ int mdbCPCreate(const mdb_config* cfg, mdb_cp** out)
     p->mass          = 1.0;
     p->dtforce       = 0.5 * p->dt;
     p->reneigh_every = cfg->reneigh_every > 0 ? cfg->reneigh_every : 20;
-    p->prune_every   = 1000;
+    p->prune_every   = cfg->prune_every > 0 ? cfg->prune_every : 1000;
     p->half_neigh    = cfg->half_neigh;
     if (cfg->lattice_a > 0) p->rho = 4.0 / (cfg->lattice_a * cfg->lattice_a * cfg->lattice_a);
     p->lattice = pow((4.0 / p->rho), (1.0 / 3.0));
@@ -197,14 +197,14 @@ int mdbCPRun(mdb_cp* s, int nsteps, double* ksum)
     for (int n = 0; n < nsteps; n++) {
         initialIntegrateClusterGFX950(p, a);
         if ((n + 1) % p->reneigh_every) {
+            if (!((n + 1) % p->prune_every)) pruneNeighborClusterGFX950(p, a, nb);
             updatePbcClusterGFX950(a, p, false);
         } else {
             updateSingleAtomsGFX950(p, a);
             updateAtomsPbcClusterGFX950(a, p, false);
             if (reneighbour(s)) return mdgfx950Status();
         }
-        computeForceLJClusterGFX950(p, a, nb, &s->stats);
-        finalIntegrateClusterGFX950(p, a);
+        computeForceFinalClusterGFX950(p, a, nb, &s->stats);
         if (ksum) ksum[n] = kineticSumClusterGFX950(p, a);
         if (mdgfx950Status()) return mdgfx950Status();
     }

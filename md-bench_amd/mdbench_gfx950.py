@@ -33,6 +33,7 @@ class Config(C.Structure):
         ("pair_epsilon", C.POINTER(C.c_double)), ("pair_sigma", C.POINTER(C.c_double)),
         ("pair_cutoff", C.POINTER(C.c_double)), ("type1_fraction", C.c_double),
         ("type_seed", C.c_int), ("input_file", C.c_char_p),
+        ("prune_every", C.c_int),
     ]
 
 
@@ -123,6 +124,8 @@ def lib():
     L.mdgfx950StatusMessage.restype = C.c_char_p
     L.mdgfx950SetStrict.argtypes = [C.c_int]
     L.mdgfx950EnableTiming.argtypes = [C.c_int]
+    L.mdgfx950TimeOnly.argtypes = [C.c_char_p]
+    L.mdgfx950TimeOnly.restype = C.c_int
     L.mdgfx950KernelTime.argtypes = [C.c_char_p, _PD, C.POINTER(C.c_longlong)]
     # drop-in entry points (opaque struct pointers from the driver)
     for fn in ("computeForceLJGFX950", "computeForceEamGFX950"):
@@ -163,6 +166,17 @@ def lib():
     L.energyVirialClusterGFX950.argtypes = [_PD, _PD]
     L.downloadClusterArrayGFX950.argtypes = [C.c_char_p, vp, C.c_longlong]
     L.downloadClusterArrayGFX950.restype = C.c_longlong
+    L.uploadClusterArrayGFX950.argtypes = [C.c_char_p, vp, C.c_longlong]
+    L.uploadClusterArrayGFX950.restype = C.c_longlong
+    L.pruneNeighborClusterGFX950.argtypes = [vp, vp, vp]
+    L.pruneNeighborClusterGFX950.restype = None
+    for fn in ("initialIntegrateClusterGFX950", "finalIntegrateClusterGFX950"):
+        getattr(L, fn).argtypes = [vp, vp]
+        getattr(L, fn).restype = None
+    L.updatePbcClusterGFX950.argtypes = [vp, vp, C.c_bool]
+    L.updatePbcClusterGFX950.restype = None
+    L.computeForceFinalClusterGFX950.argtypes = [vp, vp, vp, vp]
+    L.computeForceFinalClusterGFX950.restype = C.c_double
     L.clusterInfoGFX950.argtypes = [_ip]
     L.setClusterPrecisionGFX950.argtypes = [C.c_int, C.c_int]
     _lib = L
@@ -368,6 +382,20 @@ class CPSim:
         if n < 0:
             raise RuntimeError("download %s failed (%d): %s" % (name, n, status()[1]))
         return buf[:n].copy()
+
+    def set(self, name, arr):
+        """Overwrites a device array from host (slot_x, slot_v)."""
+        a = np.ascontiguousarray(arr, np.float64)
+        n = lib().uploadClusterArrayGFX950(name.encode(), a.ctypes.data_as(C.c_void_p), a.size)
+        if n != a.size:
+            raise RuntimeError("upload %s failed (%d): %s" % (name, n, status()[1]))
+
+    def prune(self):
+        """pruneNeighbor on the current lists and slot positions."""
+        lib().pruneNeighborClusterGFX950(self.param_p, self.atom_p, self.neighbor_p)
+        st = status()
+        if st[0]:
+            raise RuntimeError("prune failed: %s" % st[1])
 
     def compute_force(self):
         return lib().computeForceLJClusterGFX950(self.param_p, self.atom_p, self.neighbor_p,

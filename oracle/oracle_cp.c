@@ -562,6 +562,57 @@ long long orc_cp_build_neighbor(orc_cp* g, int half)
     return off;
 }
 
+/* pruneNeighborCPU: src/clusterpair/neighbor.c:1099-1264 (the plain branch; with every shape
+   here CLUSTER_N equals the reference's VECTOR_WIDTH, so no dummy padding is removed or added).
+   An entry with no atom pair inside cutneigh is overwritten by the current last entry, which is
+   tested in its place; a removal at a position inside the masked prefix shortens the prefix.
+   As in the reference, the interaction masks stay at their positions.  The CSR storage is
+   compacted afterwards (order within each list unchanged).  Returns the entries removed. */
+long long orc_cp_prune(orc_cp* g)
+{
+    const int M = g->M, N = g->N;
+    long long off = 0, out = 0, removed = 0;
+    for (int ci = 0; ci < g->nci; ci++) {
+        int n  = g->numneigh[ci];
+        int nm = g->numneigh_masked[ci];
+        int* lst      = &g->nb[off];
+        unsigned* msk = &g->imask[off];
+        int k = 0;
+        while (k < n) {
+            int cj = lst[k], in = 0;
+            for (int cii = 0; cii < M && !in; cii++) {
+                const double* xi = &g->sx[3 * (ci * M + cii)];
+                for (int cjj = 0; cjj < N; cjj++) {
+                    const double* xj = &g->sx[3 * (cj * N + cjj)];
+                    double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                    double rsq = g->fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                             : dx * dx + dy * dy + dz * dz;
+                    if (rsq < g->cutneighsq) {
+                        in = 1;
+                        break;
+                    }
+                }
+            }
+            if (in) {
+                k++;
+            } else {
+                n--;
+                if (k < nm) nm--;
+                lst[k] = lst[n];
+            }
+        }
+        removed += g->numneigh[ci] - n;
+        off += g->numneigh[ci];
+        memmove(&g->nb[out], lst, sizeof(int) * n);
+        memmove(&g->imask[out], msk, sizeof(unsigned) * n);
+        out += n;
+        g->numneigh[ci]        = n;
+        g->numneigh_masked[ci] = nm;
+    }
+    g->nb_total = out;
+    return removed;
+}
+
 /* computeForceLJRef: src/clusterpair/force_lj.c:22-189 (single type: eps, sigma6 given). f is
    per slot; energy/virial (may be NULL) are pair sums counted once per pair (new). */
 void orc_cp_force_lj(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq,
@@ -684,9 +735,11 @@ int orc_cp_reneighbour(orc_cp* g, int half)
     return 0;
 }
 
-/* Timestep loop of src/clusterpair/main.c:326-379 (single type LJ). */
-int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq, double dt,
-    double dtforce, int ntimes, int reneigh_every, double mass, double* ksum)
+/* Timestep loop of src/clusterpair/main.c:326-379 (single type LJ); prune_every > 0 prunes the
+   lists on steps between rebuilds, before the ghost update, as main.c:329-332 does. */
+int orc_cp_run_prune(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq,
+    double dt, double dtforce, int ntimes, int reneigh_every, int prune_every, double mass,
+    double* ksum)
 {
     int rc = orc_cp_reneighbour(g, half);
     if (rc) return rc;
@@ -695,6 +748,7 @@ int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutfor
     for (int n = 0; n < ntimes; n++) {
         orc_cp_initial_integrate(g, dt, dtforce);
         if ((n + 1) % reneigh_every) {
+            if (prune_every > 0 && !((n + 1) % prune_every)) orc_cp_prune(g);
             orc_cp_update_pbc(g, 0);
         } else {
             orc_cp_update_atoms(g);
@@ -706,6 +760,13 @@ int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutfor
         if (ksum) ksum[n + 1] = orc_cp_kinetic_sum(g, mass);
     }
     return 0;
+}
+
+int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq, double dt,
+    double dtforce, int ntimes, int reneigh_every, double mass, double* ksum)
+{
+    return orc_cp_run_prune(g, half, epsilon, sigma6, cutforcesq, dt, dtforce, ntimes,
+        reneigh_every, 0, mass, ksum);
 }
 
 /* ------------------------------------------------------------------ accessors for tests */

@@ -129,6 +129,7 @@ int orc_cp_setup_pbc(orc_cp* g);
 void orc_cp_update_pbc(orc_cp* g, int first);
 int orc_cp_bin_jclusters(orc_cp* g);
 long long orc_cp_build_neighbor(orc_cp* g, int half);
+long long orc_cp_prune(orc_cp* g);
 int orc_cp_reneighbour(orc_cp* g, int half);
 void orc_cp_force_lj(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq,
     double* energy, double* virial);
@@ -138,6 +139,9 @@ void orc_cp_update_atoms(orc_cp* g);
 double orc_cp_kinetic_sum(const orc_cp* g, double mass);
 int orc_cp_run(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq, double dt,
     double dtforce, int ntimes, int reneigh_every, double mass, double* ksum);
+int orc_cp_run_prune(orc_cp* g, int half, double epsilon, double sigma6, double cutforcesq,
+    double dt, double dtforce, int ntimes, int reneigh_every, int prune_every, double mass,
+    double* ksum);
 void orc_cp_info(const orc_cp* g, int* info, long long* nb_total);
 long long orc_cp_get(const orc_cp* g, const char* name, void* out, long long cap);
 void orc_cp_set_slot_x(orc_cp* g, const double* x, int nslots);

@@ -129,7 +129,18 @@ def main_cp():
                 name = "cp_%s_4%s" % (variant, "_half" if half else "")
                 run_case(exe, name, [4, 4, 4, half, 40, 20, dump_force], wd)
             run_case(exe, "cp_%s_5x6x7" % variant, [5, 6, 7, 0, 20, 10, dump_force], wd)
+            # pruneNeighbor after a fixed displacement of the first build's slots
+            run_case(exe, "cp_%s_prune" % variant, [4, 4, 4, 0, 0, 20, 0, 1], wd)
+
+
+def main_cp_prune():
+    with tempfile.TemporaryDirectory() as wd:
+        for variant in CP_VARIANTS:
+            exe = build_cp_dumper(wd, variant)
+            run_case(exe, "cp_%s_prune" % variant, [4, 4, 4, 0, 0, 20, 0, 1], wd)
 
 
 if __name__ == "__main__" and os.environ.get("MDB_GOLDEN_CP", "1") == "1":
     main_cp()
+elif __name__ == "__main__" and os.environ.get("MDB_GOLDEN_CP_PRUNE", "0") == "1":
+    main_cp_prune()

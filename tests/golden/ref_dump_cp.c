@@ -6,7 +6,11 @@
  * (src/clusterpair/main.c:43-97 setup, :99-135 reneighbour/updateAtoms, :326-379 loop).
  * Writes raw blobs into <outdir>; make_golden.py packs them.
  *
- * usage: ref_dump_cp <outdir> <nx> <ny> <nz> <half> <nsteps> <reneigh_every> <dump_force>
+ * usage: ref_dump_cp <outdir> <nx> <ny> <nz> <half> <nsteps> <reneigh_every> <dump_force> [prune]
+ *
+ * With prune = 1 the run stops after the first build: the local slots are displaced by a fixed
+ * integer pattern (up to 0.25 per coordinate), the ghosts refreshed (updatePbc) and the lists
+ * pruned (pruneNeighbor, src/clusterpair/neighbor.c:1099-1264); tag "pr" holds the result.
  */
 #include <math.h>
 #include <stdio.h>
@@ -217,6 +221,7 @@ int main(int argc, char** argv)
     param.ntimes        = atoi(argv[6]);
     param.reneigh_every = atoi(argv[7]);
     int dump_force      = atoi(argv[8]);
+    int prune_test      = argc > 9 ? atoi(argv[9]) : 0;
     param.cutneigh      = param.cutforce + param.skin;
     param.lattice       = pow((4.0 / param.rho), (1.0 / 3.0));
     param.xprd          = param.nx * param.lattice;
@@ -260,6 +265,22 @@ int main(int argc, char** argv)
     buildNeighbor(&atom, &neighbor);
     computeForce(&param, &atom, &neighbor, &stats);
     dump_clusters("s0", &atom, &neighbor, 1, dump_force);
+    if (prune_test) {
+        for (int ci = 0; ci < atom.Nclusters_local; ci++) {
+            double* cx = &atom.cl_x[CI_VECTOR3_BASE_INDEX(ci)];
+            for (int ii = 0; ii < atom.iclusters[ci].natoms; ii++) {
+                int s = ci * CLUSTER_M + ii;
+                cx[CL_X_INDEX_3D(ii)] += 0.25 * (double)((s * 7) % 9 - 4) / 4.0;
+                cx[CL_Y_INDEX_3D(ii)] += 0.25 * (double)((s * 5) % 7 - 3) / 3.0;
+                cx[CL_Z_INDEX_3D(ii)] += 0.25 * (double)((s * 3) % 5 - 2) / 2.0;
+            }
+        }
+        updatePbc(&atom, &param, false);
+        pruneNeighbor(&param, &atom, &neighbor);
+        dump_clusters("pr", &atom, &neighbor, 1, 0);
+        fclose(g_meta);
+        return 0;
+    }
 
     double* ks = malloc(sizeof(double) * (param.ntimes + 1));
     ks[0]      = ksum(&atom, &param);

@@ -262,6 +262,11 @@ class CP:
             L.orc_cp_kinetic_sum.restype = C.c_double
             L.orc_cp_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _dp]
+            L.orc_cp_run_prune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           C.c_int, C.c_double, _dp]
+            L.orc_cp_prune.argtypes = [C.c_void_p]
+            L.orc_cp_prune.restype = C.c_longlong
             L.orc_cp_info.argtypes = [C.c_void_p, _ip, C.POINTER(C.c_longlong)]
             L.orc_cp_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong]
             L.orc_cp_get.restype = C.c_longlong
@@ -312,10 +317,18 @@ class CP:
         self.L.orc_cp_force_lj(self.h, int(half), eps, sig6, cutf, C.byref(e), C.byref(w))
         return self.get("slot_f"), e.value, w.value
 
+    def set_slot_x(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        self.L.orc_cp_set_slot_x(self.h, x, x.size // 3)
+
+    def prune(self):
+        """pruneNeighborCPU on the current lists; returns the entries removed."""
+        return self.L.orc_cp_prune(self.h)
+
     def run(self, half, ntimes, reneigh, dt=0.005, dtforce=0.0025, mass=1.0, eps=1.0, sig6=1.0,
-            cutf=6.25):
+            cutf=6.25, prune_every=0):
         ks = np.zeros(ntimes + 1)
-        rc = self.L.orc_cp_run(self.h, int(half), eps, sig6, cutf, dt, dtforce, ntimes, reneigh,
-                               mass, ks)
+        rc = self.L.orc_cp_run_prune(self.h, int(half), eps, sig6, cutf, dt, dtforce, ntimes,
+                                     reneigh, prune_every, mass, ks)
         assert rc == 0, rc
         return ks

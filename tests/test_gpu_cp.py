@@ -117,6 +117,80 @@ def test_fp64_m1n2_force_and_trajectory_match_reference(size):
         sim.close()
 
 
+@pytest.mark.parametrize("variant", list(SHAPES))
+def test_prune_matches_reference(variant):
+    """pruneNeighbor on the device after the golden displacement of the first build's slots:
+    list contents, order and masked counts equal the reference's pruneNeighborCPU."""
+    print("case prune", variant, flush=True)
+    d = load_golden("cp_%s_prune" % variant)
+    sim = product().CPSim(**_cfg(d, variant))
+    try:
+        np.testing.assert_array_equal(sim.get("neighbors"), d["s0_neighbors"])
+        cur = sim.get("slot_x")
+        pr = d["pr_slot_x"]
+        assert pr.size <= cur.size
+        up = cur.copy()
+        # empty slots are INF in the reference; keep the device's far sentinels there
+        up[:pr.size] = np.where(np.isfinite(pr), pr, cur[:pr.size])
+        sim.set("slot_x", up)
+        sim.prune()
+        np.testing.assert_array_equal(sim.get("numneigh"), d["pr_numneigh"])
+        np.testing.assert_array_equal(sim.get("numneigh_masked"), d["pr_numneigh_masked"])
+        np.testing.assert_array_equal(sim.get("neighbors"), d["pr_neighbors"])
+    finally:
+        sim.close()
+
+
+def test_fp64_m1n2_trajectory_with_prune_matches_oracle():
+    """Pruning every 3 steps between rebuilds (main.c:329-332 order: before the ghost update):
+    the strict 1x2 trajectory equals the oracle's bit for bit."""
+    d = load_golden("cp_m1n2_5x6x7")
+    sim = product().CPSim(**_cfg(d, "m1n2", prune_every=3, ntimes=0))
+    try:
+        ks = sim.run(20, trace=True)
+        cjn = sim.get("cj_natoms")
+        live = _live_jslots(cjn[:sim.info()["ncj"]], 2)
+        sx = sim.get("slot_x").reshape(-1, 3)[live]
+        nb = sim.get("neighbors")
+    finally:
+        sim.close()
+    lat = meta(d, "lattice")
+    cp = ob.CP(1, 2, (5 * lat, 6 * lat, 7 * lat), meta(d, "cutneigh"), d["atoms_x"],
+               d["atoms_v"], d["atoms_type"])
+    kso = cp.run(0, 20, 10, dt=meta(d, "dt"), dtforce=meta(d, "dtforce"), prune_every=3)
+    np.testing.assert_allclose(ks, kso[1:], rtol=1e-13)
+    np.testing.assert_array_equal(sx, cp.get("slot_x").reshape(-1, 3)[live])
+    np.testing.assert_array_equal(nb, cp.get("neighbors"))
+
+
+def test_fused_force_final_equals_separate_calls():
+    """The run loop's fused force + final kick (computeForceFinalClusterGFX950) gives the same
+    FP32-mixed trajectory, bit for bit, as the reference's separate calls driven one by one:
+    initialIntegrate, updatePbc, computeForce, finalIntegrate (main.c:326-352)."""
+    P = product()
+    kw = dict(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=1000)
+    a = P.CPSim(**kw)
+    try:
+        a.run(10)
+        xa, va, fa = a.get("slot_x"), a.get("slot_v"), a.get("slot_f")
+    finally:
+        a.close()
+    b = P.CPSim(**kw)
+    try:
+        L = P.lib()
+        for _ in range(10):
+            L.initialIntegrateClusterGFX950(b.param_p, b.atom_p)
+            L.updatePbcClusterGFX950(b.atom_p, b.param_p, False)
+            b.compute_force()
+            L.finalIntegrateClusterGFX950(b.param_p, b.atom_p)
+        xb, vb, fb = b.get("slot_x"), b.get("slot_v"), b.get("slot_f")
+    finally:
+        b.close()
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(va, vb)
+    np.testing.assert_array_equal(fa, fb)
+
+
 def _thermalised_atoms(nx, steps=60):
     """A liquid-like state from the Verlet-list oracle (FP64), wrapped into the box."""
     P = product()

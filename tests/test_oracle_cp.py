@@ -92,6 +92,43 @@ def test_reneighbour_after_motion_matches_reference(variant, size):
     np.testing.assert_array_equal(cp.get("numneigh_masked"), d["r1_numneigh_masked"])
 
 
+def _csr_sets(numneigh, neighbors):
+    off = np.concatenate([[0], np.cumsum(numneigh)])
+    return [neighbors[off[c]:off[c + 1]] for c in range(numneigh.size)]
+
+
+@pytest.mark.parametrize("variant", list(SHAPES))
+def test_prune_matches_reference(variant):
+    """pruneNeighborCPU (neighbor.c:1099-1264) on the first build's lists after the golden
+    displacement of the local slots (ghosts refreshed by the reference): list contents, order,
+    masked counts and masks equal the reference's.  Independently of the reference, every entry
+    kept has an atom pair inside cutneigh and every entry dropped has none (brute force)."""
+    d = load_golden("cp_%s_prune" % variant)
+    cp, half, _ = _setup(d, variant)
+    cp.reneighbour(half)
+    np.testing.assert_array_equal(cp.get("neighbors"), d["s0_neighbors"])
+    cp.set_slot_x(d["pr_slot_x"])
+    removed = cp.prune()
+    assert removed == d["s0_numneigh"].sum() - d["pr_numneigh"].sum() > 0
+    np.testing.assert_array_equal(cp.get("numneigh"), d["pr_numneigh"])
+    np.testing.assert_array_equal(cp.get("numneigh_masked"), d["pr_numneigh_masked"])
+    np.testing.assert_array_equal(cp.get("neighbors"), d["pr_neighbors"])
+    np.testing.assert_array_equal(cp.get("imask"), d["pr_imask"].view(np.uint32))
+    M, N = SHAPES[variant]
+    xs = d["pr_slot_x"].reshape(-1, 3)
+    cut2 = meta(d, "cutneigh") ** 2
+    before = _csr_sets(d["s0_numneigh"], d["s0_neighbors"])
+    after = _csr_sets(d["pr_numneigh"], d["pr_neighbors"])
+    for ci in range(len(before)):
+        xi = xs[ci * M:(ci + 1) * M]
+        kept = set(after[ci].tolist())
+        assert kept <= set(before[ci].tolist())
+        for cj in before[ci]:
+            xj = xs[cj * N:(cj + 1) * N]
+            r2 = ((xi[:, None, :] - xj[None, :, :]) ** 2).sum(-1).min()
+            assert (r2 < cut2) == (cj in kept), (ci, cj, r2)
+
+
 @pytest.mark.parametrize("variant", ["m4n4", "m4n8"])
 @pytest.mark.parametrize("half", [0, 1])
 def test_mxn_force_equals_verlet_force(variant, half):
