@@ -552,6 +552,22 @@ def bench_copy(args, mod):
 
 import numpy as np  # noqa: E402
 
+
+def attach_traffic(res, workload):
+    """HBM bytes per launch of the reported kernel, from the two counter passes summarised in
+    profiles/ (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a plain copy
+    of known size); stays null for a workload without such a measurement."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    roof = res.get("roofline")
+    if not roof or not os.path.exists(path):
+        return
+    with open(path) as fh:
+        t = json.load(fh).get(workload)
+    if t:
+        roof["traffic"] = t["bytes"]
+        roof["traffic_source"] = "profiles/" + t["source"]
+
+
 WORKLOADS = {
     "cp-lj-64": bench_cp,
     "cp-lj-64-4x8": lambda a, m: bench_cp(a, m, N=8),
@@ -585,6 +601,7 @@ def main():
         sys.exit(2)
     mod = binding()
     res, nx = WORKLOADS[args.workload](args, mod)
+    attach_traffic(res, args.workload)
     cb = None
     if not args.no_cpu_baseline and nx is not None:
         try:
