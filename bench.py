@@ -483,6 +483,7 @@ def cpu_baseline_surrogate(model, frame, reps=3):
     p.set_flat(model.theta.astype(np.float64))
     p.d_mean, p.d_std = model.d_mean.astype(np.float64), model.d_std.astype(np.float64)
     p.e0, p.s_e = model.e0, model.s_e
+    p.r_core, p.k_core = model.r_core, model.k_core
     t0 = time.perf_counter()
     for _ in range(reps):
         so.energy_and_forces(p, frame["x"], frame["nlocal"], frame["owner"], frame["numneigh"],

@@ -67,6 +67,16 @@ int surEvaluateGFX950(int n, const float* dn, const float* mom, float* e, void* 
 int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, const int* npairs,
     const void* coef, double* f);
 
+/* Short-range core pair term U(r) = k_core (r_core - r)^3 for r < r_core (0 beyond), added to
+   the network's energy: it keeps atoms apart below the closest approach the training data
+   holds, where the network has no data.  surSetCoreGFX950 sets it for surForcesGFX950 and the
+   drop-in force call (default k_core = 0: no core); surCoreGFX950 computes its per-atom
+   energies e (device, n) and forces f (device, n x 3) alone, in FP64, over the Verlet list
+   (training labels have it subtracted).  e or f may be NULL. */
+int surSetCoreGFX950(float r_core, float k_core);
+int surCoreGFX950(const MdgSurSystem* sys, int i0, int n, float r_core, float k_core, double* e,
+    double* f);
+
 /* The drop-in force call computeForceSurrogateGFX950 and the judge surrogateJudgeGFX950 work on
    the Verlet-list structs and are declared in mdgfx950.h. */
 

@@ -55,6 +55,7 @@ struct SurModel {
     bf16x8* frag  = nullptr;  // device, NFRAG * 64
     float* norm   = nullptr;  // device, mean[32], inv_std[32]
     float e0 = 0.f, s_e = 1.f;
+    float r_core = 0.f, k_core = 0.f;  // short-range core U(r) = k (r_core - r)^3, r < r_core
     bool ready = false;
     // per-atom work arrays of the drop-in force call
     float *dn = nullptr, *mom = nullptr;
@@ -531,7 +532,8 @@ __device__ inline void load_block(const __half* __restrict__ p, float* c)
 
 __global__ void __launch_bounds__(BLOCK) k_sur_force(int nlocal, int i0, int n,
     const double* __restrict__ x, const int* __restrict__ border_map, const int* __restrict__ cl,
-    const int* __restrict__ ncl, const __half* __restrict__ coef, double* __restrict__ f)
+    const int* __restrict__ ncl, const __half* __restrict__ coef, float rcore, float kcore,
+    double* __restrict__ f)
 {
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
     const int a    = gid / LPA;
@@ -579,7 +581,9 @@ __global__ void __launch_bounds__(BLOCK) k_sur_force(int nlocal, int i0, int n,
             const float my  = sxy * p.ux + syy * p.uy + syz * p.uz;
             const float mz  = sxz * p.ux + syz * p.uy + szz * p.uz;
             const float uMu = p.ux * mx + p.uy * my + p.uz * mz;
-            const float sa  = S + dw * (ua + uMu);
+            float sa        = S + dw * (ua + uMu);
+            // the core pair term, once per pair (lane 0): F_i += U'(r) u, U' = -3 k (rc - r)^2
+            if (q == 0 && p.r < rcore) sa -= 3.0f * kcore * (rcore - p.r) * (rcore - p.r);
             const float w2  = 2.0f * wr;
             fx += sa * p.ux + wr * (dax - ua * p.ux) + w2 * (mx - uMu * p.ux);
             fy += sa * p.uy + wr * (day - ua * p.uy) + w2 * (my - uMu * p.uy);
@@ -590,6 +594,40 @@ __global__ void __launch_bounds__(BLOCK) k_sur_force(int nlocal, int i0, int n,
     fy = gsum<LPA>(fy);
     fz = gsum<LPA>(fz);
     if (act && q == 0) {
+        f[3 * (size_t)a]     = fx;
+        f[3 * (size_t)a + 1] = fy;
+        f[3 * (size_t)a + 2] = fz;
+    }
+}
+
+// ---------------------------------------------------------------- the core pair term alone
+// Per-atom energy E_i = 1/2 sum_j U(r_ij) and force F_i = sum_j U'(r_ij) u_ij over the full
+// Verlet list (FP64), U(r) = k (r_core - r)^3 for r < r_core: the part of the model the network
+// does not learn (training labels have it subtracted).
+__global__ void k_sur_core(int i0, int n, const double* __restrict__ x,
+    const int* __restrict__ neighbors, const int* __restrict__ numneigh, int nlocal, double rcore,
+    double kcore, double* __restrict__ e, double* __restrict__ f)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int i = i0 + a;
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    double ea = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    const int nn = numneigh[i];
+    for (int k = 0; k < nn; k++) {
+        const int j    = neighbors[(size_t)k * nlocal + i];
+        const double dx = x[3 * j] - xi, dy = x[3 * j + 1] - yi, dz = x[3 * j + 2] - zi;
+        const double r  = sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(r < rcore) || r == 0.0) continue;
+        const double t  = rcore - r;
+        ea += 0.5 * kcore * t * t * t;
+        const double du = -3.0 * kcore * t * t / r;  // U'(r) / r
+        fx += du * dx;
+        fy += du * dy;
+        fz += du * dz;
+    }
+    if (e) e[a] = ea;
+    if (f) {
         f[3 * (size_t)a]     = fx;
         f[3 * (size_t)a + 1] = fy;
         f[3 * (size_t)a + 2] = fz;
@@ -1062,8 +1100,35 @@ int surForcesGFX950(const MdgSurSystem* sys, int i0, int n, const int* pairs, co
     }
     TimedRegion tr(F_SURROGATE);
     k_sur_force<<<blocks_for((long long)n * LPA, BLOCK), BLOCK, 0, stream()>>>(sys->nlocal, i0, n,
-        sys->x, sys->border_map, pairs, npairs, (const __half*)coef, f);
+        sys->x, sys->border_map, pairs, npairs, (const __half*)coef, M.r_core, M.k_core, f);
     MDG_LAUNCH_CHECK("sur_force");
+    return failed() ? -1 : 0;
+}
+
+int surSetCoreGFX950(float r_core, float k_core)
+{
+    if (!(r_core >= 0.f) || !(r_core < RC) || !(k_core >= 0.f)) {
+        set_error(E_ARG, "surSetCoreGFX950: r_core %g (must be in [0, %g)), k_core %g", r_core,
+            RC, k_core);
+        return -1;
+    }
+    M.r_core = r_core;
+    M.k_core = k_core;
+    return 0;
+}
+
+int surCoreGFX950(const MdgSurSystem* sys, int i0, int n, float r_core, float k_core, double* e,
+    double* f)
+{
+    if (failed() || !stream()) return -1;
+    if (n <= 0) return 0;
+    if (i0 < 0 || i0 + n > sys->nlocal) {
+        set_error(E_ARG, "surCoreGFX950: range %d+%d outside %d atoms", i0, n, sys->nlocal);
+        return -1;
+    }
+    k_sur_core<<<blocks_for(n, BLOCK), BLOCK, 0, stream()>>>(i0, n, sys->x, sys->neighbors,
+        sys->numneigh, sys->nlocal, (double)r_core, (double)k_core, e, f);
+    MDG_LAUNCH_CHECK("sur_core");
     return failed() ? -1 : 0;
 }
 

@@ -28,6 +28,11 @@ ND, NMOM, NCOEF, NPARAM = 32, 36, 64, 6337
 H = 64
 RC = 4.95
 A_CU = 3.615
+# Core pair term U(r) = K_CORE (R_CORE - r)^3 below R_CORE, outside what the network learns: the
+# training snapshots hold no pair closer than about 1.93 A, and without it a pair that wanders
+# below that range meets no repulsion (the network's extrapolation) and collapses.
+R_CORE = 2.0   # A
+K_CORE = 30.0  # eV / A^3
 
 
 def _binding():
@@ -79,12 +84,14 @@ def lib():
     L.surForceRmseGFX950.argtypes = [i, vp, vp]
     L.surForceRmseGFX950.restype = d
     L.surMfmaProbeGFX950.argtypes = [vp, vp, vp]
+    L.surSetCoreGFX950.argtypes = [f, f]
+    L.surCoreGFX950.argtypes = [sp, i, i, f, f, vp, vp]
     L.mdbVLUseSurrogate.argtypes = [vp, i, i, d]
     L.mdbVLSurrogateStats.argtypes = [vp, vp]
     for fn in ("surSetModelGFX950", "surSetParamsDeviceGFX950", "surGetParamsGFX950",
                "surDescriptorsGFX950", "surEvaluateGFX950", "surForcesGFX950",
                "surLossTermsGFX950", "surAdjointGFX950", "surWeightGradGFX950", "surAdamGFX950",
-               "surMfmaProbeGFX950", "mdbVLUseSurrogate"):
+               "surMfmaProbeGFX950", "mdbVLUseSurrogate", "surSetCoreGFX950", "surCoreGFX950"):
         getattr(L, fn).restype = i
     _bound = True
     return L
@@ -224,6 +231,17 @@ class PackedFrames:
         self.e_host, self.f_host = e, f
         self.sys = SurSystem(N, self.nghost, self.x.ptr, self.bmap.ptr, self.nb.ptr, self.nn.ptr)
 
+    def subtract_core(self, r_core, k_core):
+        """Labels minus the core pair term (computed on the device over the packed lists)."""
+        N = self.nlocal
+        ec, fc = Dev(N, np.float64), Dev(3 * N, np.float64)
+        _check(lib().surCoreGFX950(C.byref(self.sys), 0, N, r_core, k_core, ec.ptr, fc.ptr),
+               "surCoreGFX950")
+        self.e_host = self.e_host - ec.get()
+        self.f_host = self.f_host - fc.get()
+        self.e_ref = Dev.of(self.e_host, np.float64)
+        self.f_ref = Dev.of(self.f_host, np.float64)
+
     def frame_range(self, a, b):
         """Atom range [i0, i1) of frames a .. b-1."""
         return int(self.loff[a]), int(self.loff[b])
@@ -242,26 +260,32 @@ def init_theta(seed=1):
 
 
 class Model:
-    """Weights and normalisation of the network (host copies)."""
+    """Weights and normalisation of the network (host copies) and the core pair term."""
 
-    def __init__(self, theta, d_mean, d_std, e0, s_e):
+    def __init__(self, theta, d_mean, d_std, e0, s_e, r_core=R_CORE, k_core=K_CORE):
         self.theta = np.asarray(theta, np.float32)
         self.d_mean = np.asarray(d_mean, np.float32)
         self.d_std = np.asarray(d_std, np.float32)
         self.e0, self.s_e = float(e0), float(s_e)
+        self.r_core, self.k_core = float(r_core), float(k_core)
 
-    def upload(self):
+    def upload(self, core=True):
+        """Network and normalisation to the device; core=False leaves the core term out of the
+        force call (training fits the network to labels with the core subtracted)."""
         _check(lib().surSetModelGFX950(_p(self.theta), _p(self.d_mean), _p(self.d_std),
                                        self.e0, self.s_e), "surSetModelGFX950")
+        _check(lib().surSetCoreGFX950(self.r_core, self.k_core if core else 0.0),
+               "surSetCoreGFX950")
 
     def save(self, path):
         np.savez(path, theta=self.theta, d_mean=self.d_mean, d_std=self.d_std,
-                 e0=self.e0, s_e=self.s_e)
+                 e0=self.e0, s_e=self.s_e, r_core=self.r_core, k_core=self.k_core)
 
     @classmethod
     def load(cls, path):
         z = np.load(path, allow_pickle=False)
-        return cls(z["theta"], z["d_mean"], z["d_std"], float(z["e0"]), float(z["s_e"]))
+        core = (float(z["r_core"]), float(z["k_core"])) if "k_core" in z.files else (0.0, 0.0)
+        return cls(z["theta"], z["d_mean"], z["d_std"], float(z["e0"]), float(z["s_e"]), *core)
 
 
 class Workspace:
@@ -324,7 +348,7 @@ class Trainer:
         self.m.zero()
         self.v.zero()
         self.t = 0
-        model.upload()
+        model.upload(core=False)
 
     def _max_batch_atoms(self, pk):
         b = self.batch_frames
@@ -406,6 +430,8 @@ def train_surrogate(temps=None, per_temp=100, every=50, nx=10, epochs=30, batch_
     nval = max(1, int(round(valid_fraction * len(frames))))
     valid = PackedFrames([frames[k] for k in order[:nval]])
     train = PackedFrames([frames[k] for k in order[nval:]])
+    for pk in (train, valid):
+        pk.subtract_core(R_CORE, K_CORE)
     ws0 = Workspace(int(max(f["nlocal"] for f in frames)), train.maxn)
     d_mean, d_std = descriptor_stats(train, range(0, train.nframes, max(1, train.nframes // 64)),
                                      ws0)

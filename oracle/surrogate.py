@@ -11,7 +11,7 @@ define the model the GPU kernels compute, and every one is pinned by what the ma
                        invariance
   mlp_forward/_grad    the input gradient against central differences of the energy
   forces               against central differences of the total energy sum_i E_i (ghost images
-                       moved with their owners)
+                       moved with their owners), with and without the core pair term
   force_adjoint,
   weight_gradients     against central differences of the loss in every parameter block
   adam_step            the textbook update (Kingma and Ba, 2015, Algorithm 1)
@@ -138,6 +138,9 @@ class Params:
         self.d_std = np.ones(ND) if d_std is None else np.asarray(d_std, np.float64)
         self.e0 = float(e0)
         self.s_e = float(s_e)
+        # the core pair term (not trained; k_core = 0: none)
+        self.r_core = 0.0
+        self.k_core = 0.0
 
     def flat(self):
         return np.concatenate([getattr(self, n).reshape(-1) for n in self.NAMES])
@@ -154,6 +157,7 @@ class Params:
         for n in self.NAMES + ("d_mean", "d_std"):
             setattr(p, n, getattr(self, n).copy())
         p.e0, p.s_e = self.e0, self.s_e
+        p.r_core, p.k_core = self.r_core, self.k_core
         return p
 
 
@@ -251,11 +255,28 @@ def forces(coef, x, nlocal, owner, numneigh, neighbors):
     return F
 
 
+def core_energy_forces(x, nlocal, numneigh, neighbors, r_core, k_core):
+    """The core pair term U(r) = k (r_core - r)^3 for r < r_core, 0 beyond (C2 at r_core): per
+    atom E_i = 1/2 sum_j U(r_ij) and F_i = sum_j U'(r_ij) u_ij over the full list."""
+    i, j, d, r = _pairs(x, nlocal, numneigh, neighbors)
+    m = (r < r_core) & (r > 0)
+    i, d, r = i[m], d[m], r[m]
+    t = r_core - r
+    E = np.zeros(nlocal)
+    np.add.at(E, i, 0.5 * k_core * t ** 3)
+    F = np.zeros((nlocal, 3))
+    np.add.at(F, i, (-3.0 * k_core * t * t / r)[:, None] * d)
+    return E, F
+
+
 def energy_and_forces(p, x, nlocal, owner, numneigh, neighbors, bf16_inputs=False):
     D, A, T = descriptors(x, nlocal, numneigh, neighbors)
     E, cache = mlp_forward(p, D, bf16_inputs)
     g = mlp_input_grad(p, cache, bf16_inputs)
     F = forces(coefficients(g, A, T), x, nlocal, owner, numneigh, neighbors)
+    if getattr(p, "k_core", 0.0) > 0.0:
+        Ec, Fc = core_energy_forces(x, nlocal, numneigh, neighbors, p.r_core, p.k_core)
+        E, F = E + Ec, F + Fc
     return E, F
 
 
@@ -343,6 +364,9 @@ def loss_gradient(p, frames, wE, wF):
         E, cache = mlp_forward(p, D)
         g = mlp_input_grad(p, cache)
         F = forces(coefficients(g, A, T), x, nl, ow, nn, nb)
+        if getattr(p, "k_core", 0.0) > 0.0:  # the core: constant in the parameters
+            Ec, Fc = core_energy_forces(x, nl, nn, nb, p.r_core, p.k_core)
+            E, F = E + Ec, F + Fc
         dF = F - fr["f_ref"].reshape(-1, 3)
         ug = (wF / (3 * n)) * force_adjoint(x, nl, ow, nn, nb, A, T, dF)
         cE = (2.0 * wE / n) * (E - fr["e_ref"])

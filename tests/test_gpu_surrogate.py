@@ -42,7 +42,27 @@ def _params_for(fr, seed=1):
 
 
 def _model(es, p):
-    return es.Model(p.flat().astype(np.float32), p.d_mean, p.d_std, p.e0, p.s_e)
+    return es.Model(p.flat().astype(np.float32), p.d_mean, p.d_std, p.e0, p.s_e, p.r_core,
+                    p.k_core)
+
+
+def _squeeze(frame, r_pair=1.7):
+    """The frame with atom 0's nearest neighbour moved along the bond to r_pair (ghost images
+    follow their owners)."""
+    fr = dict(frame)
+    nl = fr["nlocal"]
+    x = fr["x"].reshape(-1, 3).copy()
+    own = np.concatenate([np.arange(nl), fr["owner"]])
+    shift = x - x[own]
+    off = np.concatenate([[0], np.cumsum(fr["numneigh"])])
+    j = fr["neighbors"][off[0]:off[1]]
+    d = np.linalg.norm(x[j] - x[0], axis=1)
+    k = own[j[np.argmin(d)]]
+    u = (x[j[np.argmin(d)]] - x[0]) / d.min()
+    x[k] -= (d.min() - r_pair) * u
+    x = x[own] + shift
+    fr["x"] = x.reshape(-1)
+    return fr
 
 
 def _lane_blocks(c):
@@ -131,6 +151,38 @@ def test_network_and_forces_match_oracle(frame):
     assert _rel(f, Fb) < 2e-3
     assert _rel(f, F64) < 5e-2
     assert np.abs(f).max() > 1e-2
+
+
+def test_core_term_matches_oracle(frame):
+    """The core pair term on a frame with a pair squeezed to 1.7 A: surCoreGFX950 (FP64) equals
+    the oracle's, and the force call with the core set equals the network forces plus the core
+    (bfloat16 oracle, 2e-3)."""
+    es = surrogate_module()
+    fr = _squeeze(frame)
+    p = _params_for(frame)
+    p.r_core, p.k_core = 2.0, 30.0
+    pk = es.PackedFrames([fr])
+    n = pk.nlocal
+    L = es.lib()
+    import ctypes as C
+    ec, fc = es.Dev(n, np.float64), es.Dev(3 * n, np.float64)
+    assert L.surCoreGFX950(C.byref(pk.sys), 0, n, 2.0, 30.0, ec.ptr, fc.ptr) == 0
+    Ec, Fc = so.core_energy_forces(fr["x"], n, fr["numneigh"], fr["neighbors"], 2.0, 30.0)
+    assert np.abs(Fc).max() > 1.0
+    np.testing.assert_allclose(ec.get(n), Ec, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(fc.get(3 * n).reshape(n, 3), Fc, rtol=1e-12, atol=1e-12)
+    ws = es.Workspace(n, pk.maxn)
+    _model(es, p).upload()
+    es.predict(pk.sys, 0, n, ws)
+    f = ws.f.get(3 * n).reshape(n, 3)
+    args = (fr["x"], n, fr["owner"], fr["numneigh"], fr["neighbors"])
+    D, A, T = so.descriptors(fr["x"], n, fr["numneigh"], fr["neighbors"])
+    _, cache = so.mlp_forward(p, D, bf16_inputs=True)
+    Fb = so.forces(so.coefficients(so.mlp_input_grad(p, cache, bf16_inputs=True), A, T), *args)
+    assert _rel(f, Fb + Fc) < 2e-3
+    _model(es, p).upload(core=False)
+    es.predict(pk.sys, 0, n, ws)
+    assert _rel(ws.f.get(3 * n).reshape(n, 3), Fb) < 2e-3
 
 
 def test_training_gradients_match_oracle(frame):

@@ -155,6 +155,60 @@ def test_forces_are_minus_energy_gradient(kind):
             assert abs(fd - F[k, c]) <= 1e-6 * max(1.0, np.abs(F).max()), (k, c, fd, F[k, c])
 
 
+def squeezed_frame(seed=5, r_pair=1.7):
+    """A periodic frame with one pair pushed to r_pair (inside a 2.0 A core)."""
+    fr = periodic_frame(seed=seed)
+    nl = fr["nlocal"]
+    x = fr["x"].reshape(-1, 3).copy()
+    off = np.concatenate([[0], np.cumsum(fr["numneigh"])])
+    j = fr["neighbors"][off[0]:off[1]]
+    d = np.linalg.norm(x[j] - x[0], axis=1)
+    k = j[np.argmin(d)]
+    k_own = k if k < nl else fr["owner"][k - nl]
+    u = (x[k] - x[0]) / d.min()
+    x[k_own] = x[k_own] - (d.min() - r_pair) * u  # move the neighbour's owner along the bond
+    fr["x"] = ghosts(fr, x[:nl].reshape(-1))
+    return fr
+
+
+def test_core_term_is_c2_and_zero_beyond_rcore():
+    """U(r) = k (rc - r)^3: value, slope and curvature vanish at rc and stay 0 beyond."""
+    x = np.array([[0, 0, 0], [0, 0, 0.0]], np.float64)
+    nn, nb = np.array([1, 1], np.int32), np.array([1, 0], np.int32)
+    for r, e_exp in ((2.5, 0.0), (2.0, 0.0), (1.999, 0.5 * 30 * 1e-9), (1.5, 0.5 * 30 * 0.125)):
+        x[1, 2] = r
+        E, F = so.core_energy_forces(x.reshape(-1), 2, nn, nb, 2.0, 30.0)
+        np.testing.assert_allclose(E, [e_exp, e_exp], rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(F[0], -F[1], rtol=0, atol=1e-15)
+    x[1, 2] = 1.5
+    _, F = so.core_energy_forces(x.reshape(-1), 2, nn, nb, 2.0, 30.0)
+    assert F[0, 2] == pytest.approx(-3 * 30 * 0.25)  # pushed away from the neighbour at +z
+
+
+def test_forces_with_core_are_minus_energy_gradient():
+    """The network plus the core pair term on a frame with a pair at 1.7 A: central
+    differences of the total energy."""
+    fr = squeezed_frame()
+    p = trained_like_params(periodic_frame())
+    p.r_core, p.k_core = 2.0, 30.0
+    nl = fr["nlocal"]
+    E, F = so.energy_and_forces(p, fr["x"], nl, fr["owner"], fr["numneigh"], fr["neighbors"])
+    Ec, Fc = so.core_energy_forces(fr["x"], nl, fr["numneigh"], fr["neighbors"], 2.0, 30.0)
+    assert Ec.sum() > 0.1 and np.abs(Fc).max() > 1.0
+    h = 1e-5
+    for k in [0, int(np.argmax(np.abs(Fc).sum(1))), nl - 1]:
+        for c in range(3):
+            es = []
+            for sgn in (1, -1):
+                x = fr["x"].copy()
+                x[3 * k + c] += sgn * h
+                x = ghosts(fr, x)
+                es.append(so.energy_and_forces(p, x, nl, fr["owner"], fr["numneigh"],
+                                               fr["neighbors"])[0].sum())
+            fd = -(es[0] - es[1]) / (2 * h)
+            assert abs(fd - F[k, c]) <= 1e-6 * max(1.0, np.abs(F).max()), (k, c, fd, F[k, c])
+
+
 def _frames_with_labels(n=2):
     frs = []
     for s in range(n):
