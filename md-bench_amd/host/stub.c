@@ -40,6 +40,8 @@ int mdbStubRun(int pattern, int natoms, int nneighs, int nreps, int ntimes, int 
     if (pattern < P_SEQ || pattern > P_RAND || natoms <= 0 || nneighs <= 0 || nreps <= 0)
         return MDG_ERR_ARG;
     if (pattern == P_RAND && natoms <= nneighs) return MDG_ERR_ARG;
+    /* every listed index must name an atom: the fixed pattern lists atoms 0..nneighs-1 */
+    if (pattern == P_FIX && natoms < nneighs) return MDG_ERR_ARG;
     Parameter p;
     Atom a;
     Neighbor nb;
@@ -86,10 +88,12 @@ int mdbStubRun(int pattern, int natoms, int nneighs, int nreps, int ntimes, int 
         a.x[3 * i + 1] = (double)i * 0.00001;
         a.x[3 * i + 2] = (double)i * 0.00001;
     }
-    /* createNeighbors (main-stub.c:60-110) */
+    /* createNeighbors (main-stub.c:60-110).  main-stub.c:74 starts the last atom's sequential
+     * list at index Nlocal, one past the last atom, which reads whatever lies beyond the position
+     * array; the start wraps to atom 0 here, so every listed index names an atom. */
     for (int i = 0; i < natoms; i++) {
         int* li = nb.neighbors + (size_t)i * maxn;
-        int j   = (pattern == P_SEQ) ? (i + 1) : 0;
+        int j   = (pattern == P_SEQ) ? (i + 1) % natoms : 0;
         int m   = (pattern == P_SEQ) ? natoms : nneighs;
         for (int k = 0; k < nneighs; k++) {
             if (pattern == P_RAND) {

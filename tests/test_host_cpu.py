@@ -43,6 +43,16 @@ def test_park_miller_known_answer():
     assert s.value == 1043618065
 
 
+@pytest.mark.parametrize("pattern", ["fix", "rand"])
+def test_synthetic_pattern_rejects_lists_naming_missing_atoms(pattern):
+    # 10 atoms cannot hold 76 distinct neighbours (rand) or the atoms 0..75 (fix); the call
+    # returns an argument error before any device work.
+    P = product()
+    out = np.zeros(5)
+    rc = P.lib().mdbStubRun(P.STUB_PATTERNS[pattern], 10, 76, 1, 1, 0, 1, out)
+    assert rc != 0 and not out.any()
+
+
 @pytest.mark.parametrize("case", ["vl_lj_4", "vl_lj_5x6x7", "vl_lj_types"])
 def test_lattice_and_velocities_match_reference(case):
     d = load_golden(case)
