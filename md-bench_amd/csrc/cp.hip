@@ -1241,9 +1241,12 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
 // each lane keeps the forces of its group's four i-atoms; the 16 lanes of a group reduce them at
 // the end.  The four lists of a wavefront run to the longest; shorter ones read the dummy
 // j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and arithmetic
-// are those of k_cp_force_f32.
+// are those of k_cp_force_f32, without its Newton step.  The plain variant (one type, no energy,
+// no compensation) is held to 80 VGPRs, six wavefronts per SIMD: one value spills around the
+// loops, none inside them; the others would spill inside them and keep their register count.
 template <int N, bool EV, bool UNIFORM, bool COMP>
-__global__ void __launch_bounds__(BLOCK) k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((UNIFORM && !COMP && !EV) ? 6 : 1)))
+k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
     const int* __restrict__ nb, const int* __restrict__ numneigh,
     const int* __restrict__ ci_natoms, int maxneighs, float* __restrict__ f,
@@ -1324,9 +1327,10 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32_full(int nci, int ncj, i
                 const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
                 if (below && !in) in = rsqd < cut2;
             }
-            in        = in && rsq > 0.0f;  // the self pair is the only one at distance 0
-            float r2i = __builtin_amdgcn_rcpf(rsq);
-            r2i       = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);
+            in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
+            // v_rcp_f32 alone: with a Newton step the 1M-atom force error is 5.7e-7, without it
+            // 6.0e-7 (measured), the step costs two of the ~24 instructions of a pair
+            const float r2i = __builtin_amdgcn_rcpf(rsq);
             const float r6i = r2i * r2i * r2i;
             float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
             fpair           = in ? fpair : 0.0f;
@@ -1396,7 +1400,11 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32_full(int nci, int ncj, i
             }
         }
     }
-    if (live && sub < 4 && sub < ci_natoms[ci]) {
+    // the i-cluster index again, from the wavefront index (a scalar) and the lane id, rather
+    // than kept in registers across the loops
+    const int lane_e = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const int ci_e   = wv * 4 + (lane_e >> 4);
+    if (ci_e < nci && sub < 4 && sub < ci_natoms[ci_e]) {
         float rx = 0.f, ry = 0.f, rz = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -1405,7 +1413,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32_full(int nci, int ncj, i
                 ry = COMP ? fy[a] + cy[a] : fy[a];
                 rz = COMP ? fz[a] + cz[a] : fz[a];
             }
-        const int si  = ci * 4 + sub;
+        const int si  = ci_e * 4 + sub;
         f[3 * si]     = rx;
         f[3 * si + 1] = ry;
         f[3 * si + 2] = rz;
