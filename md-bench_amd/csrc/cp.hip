@@ -1067,8 +1067,7 @@ struct CPAcc {
         }
         if (HALF) in = in && !(J.cj == ci_cj0 && !(si < J.sj));
         else in = in && rsq > 0.0f;
-        float r2i = __builtin_amdgcn_rcpf(rsq);
-        r2i       = fmaf(r2i, fmaf(-rsq, r2i, 1.0f), r2i);  // one Newton step
+        const float r2i = __builtin_amdgcn_rcpf(rsq);  // no Newton step (see the full-list kernel)
         const float r6i = r2i * r2i * r2i;
         float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
         fpair           = in ? fpair : 0.0f;
@@ -1241,9 +1240,9 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
 // each lane keeps the forces of its group's four i-atoms; the 16 lanes of a group reduce them at
 // the end.  The four lists of a wavefront run to the longest; shorter ones read the dummy
 // j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and arithmetic
-// are those of k_cp_force_f32, without its Newton step.  The plain variant (one type, no energy,
-// no compensation) is held to 80 VGPRs, six wavefronts per SIMD: one value spills around the
-// loops, none inside them; the others would spill inside them and keep their register count.
+// are those of k_cp_force_f32.  The one-type variant is held to 80 VGPRs, six wavefronts per
+// SIMD, without spills; the type-table, energy and compensated variants keep their register
+// count (held to 80 they spill inside the loop; held to 96 the type-table one gains nothing).
 template <int N, bool EV, bool UNIFORM, bool COMP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((UNIFORM && !COMP && !EV) ? 6 : 1)))
 k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
@@ -1329,7 +1328,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             }
             in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
             // v_rcp_f32 alone: with a Newton step the 1M-atom force error is 5.7e-7, without it
-            // 6.0e-7 (measured), the step costs two of the ~24 instructions of a pair
+            // 6.0e-7 (measured); the step cost two of the ~25 instructions of a pair
             const float r2i = __builtin_amdgcn_rcpf(rsq);
             const float r6i = r2i * r2i * r2i;
             float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
