@@ -168,6 +168,11 @@ def bench_cp(args, mod, N=4, half=0):
     try:
         natoms = 4 * nx ** 3
         sim.run(args.warmup)
+        # total energy per atom (potential from one energy-on force call at the same positions,
+        # kinetic 1/2 sum m v^2) before and after the timed steps, outside the timed region
+        def etot():
+            return (sim.energy_virial()[0] + 0.5 * sim.kinetic_sum()) / natoms
+        e_start = etot()
         L = mod.lib()
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"force_cp")  # events around the reported kernel only
@@ -181,6 +186,7 @@ def bench_cp(args, mod, N=4, half=0):
         L.mdgfx950KernelTime(b"force_cp", C.byref(ms), C.byref(nl))
         L.mdgfx950EnableTiming(0)
         L.mdgfx950TimeOnly(None)
+        e_end = etot()
         info = sim.info()
         pairs = int(sim.get("numneigh").astype(np.int64).sum())
         nci = info["nci"]
@@ -204,7 +210,10 @@ def bench_cp(args, mod, N=4, half=0):
                                    "cells (%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20"
                                    % (wl, N, "half" if half else "full", natoms),
                        "atoms": natoms, "i_clusters": nci,
-                       "j_clusters_per_i_cluster": pairs / nci},
+                       "j_clusters_per_i_cluster": pairs / nci,
+                       "energy_per_atom": {"start": e_start, "end": e_end,
+                                           "drift": e_end - e_start,
+                                           "over_steps": args.steps}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                          "kernel": ("k_cp_force_f32 (one wavefront per i-cluster, atomic "
