@@ -51,6 +51,10 @@ int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches
 /* Restricts event recording to one family (NULL: all families, the default), so that timing a
    kernel adds no events around the others; returns -1 for an unknown family. */
 int mdgfx950TimeOnly(const char* family);
+/* Records the events on every `every`-th entry of a timed kernel family only (default 1: all),
+   so that a long run is sampled at a fraction of the event overhead; launches counts the sampled
+   entries. */
+void mdgfx950TimeEvery(int every);
 /* Wait for all queued work. */
 void mdgfx950Synchronize(void);
 

@@ -76,6 +76,18 @@ double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neigh
    counterpart: a fused form of the two reference calls) */
 double computeForceFinalClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
     Stats* stats);
+/* computeForce, finalIntegrate and the NEXT step's initialIntegrate (main.c:349-350, 326) in one
+   call: the FP32 full-list kernel kicks each atom twice with the force it has just written and
+   drifts it into a second position buffer, which then becomes the current one; the same FP64
+   operations in the same order as the three calls, so the trajectory is bit-identical.  The
+   velocities it leaves are the next half step's: callers that read full-step velocities after a
+   step (thermo, the last step) use computeForceFinalClusterGFX950 there.  Ghost slots keep
+   positions one step older than after the separate calls until updatePbcClusterGFX950, so a
+   pruneNeighborClusterGFX950 between the two differs; callers that prune there use the separate
+   calls for the step before.  Other configurations
+   run the calls in order (no reference counterpart: a fused form of three reference calls) */
+double computeForceFinalInitialClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats);
 /* replace cudaInitialIntegrate / cudaFinalIntegrate (force_lj_cuda.cu:456-542; this one uses
    dtforce in the final kick, where the reference's CUDA final kick uses dt) */
 void initialIntegrateClusterGFX950(Parameter* param, Atom* atom);

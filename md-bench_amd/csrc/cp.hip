@@ -79,6 +79,11 @@ struct CPState {
     double *d_ci_bbox = nullptr, *d_cj_bbox = nullptr;
     size_t cap_ci_natoms = 0, cap_ci_bin = 0, cap_cj_natoms = 0, cap_ci_bbox = 0, cap_cj_bbox = 0;
     double *d_sx = nullptr, *d_sv = nullptr, *d_sf64 = nullptr;
+    // second slot-position buffer: the force call that also integrates the next step writes
+    // the new positions there (its other waves still read the current ones) and the two swap
+    double* d_sxn   = nullptr;
+    size_t cap_sxn  = 0;
+    bool sxn_valid  = false;  // padding, dummy and ghost slots of d_sxn match d_sx
     float* d_sf32 = nullptr;
     int* d_st     = nullptr;
     size_t cap_sx = 0, cap_sv = 0, cap_sf64 = 0, cap_sf32 = 0, cap_st = 0;
@@ -1249,7 +1254,8 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
     const int* __restrict__ nb, const int* __restrict__ numneigh,
     const int* __restrict__ ci_natoms, int maxneighs, float* __restrict__ f,
-    double* __restrict__ ev, double* __restrict__ v, double dtforce)
+    double* __restrict__ ev, double* __restrict__ v, double dtforce, double* __restrict__ xn,
+    double dt)
 {
     constexpr int CPG = 16 / N;  // j-clusters per group per iteration
     __shared__ float s_tab[4][MAXT2];
@@ -1405,12 +1411,16 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     const int ci_e   = wv * 4 + (lane_e >> 4);
     if (ci_e < nci && sub < 4 && sub < ci_natoms[ci_e]) {
         float rx = 0.f, ry = 0.f, rz = 0.f;
+        double px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll
         for (int a = 0; a < 4; a++)
             if (a == sub) {
                 rx = COMP ? fx[a] + cx[a] : fx[a];
                 ry = COMP ? fy[a] + cy[a] : fy[a];
                 rz = COMP ? fz[a] + cz[a] : fz[a];
+                px = xi[a][0];
+                py = xi[a][1];
+                pz = xi[a][2];
             }
         const int si  = ci_e * 4 + sub;
         f[3 * si]     = rx;
@@ -1418,9 +1428,22 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         f[3 * si + 2] = rz;
         if (v) {
             // finalIntegrate (integrate.c:61-86) of this atom, fused: its force is complete
-            v[3 * si]     = v[3 * si] + dtforce * (double)rx;
-            v[3 * si + 1] = v[3 * si + 1] + dtforce * (double)ry;
-            v[3 * si + 2] = v[3 * si + 2] + dtforce * (double)rz;
+            double vx = v[3 * si] + dtforce * (double)rx;
+            double vy = v[3 * si + 1] + dtforce * (double)ry;
+            double vz = v[3 * si + 2] + dtforce * (double)rz;
+            if (xn) {
+                // and the next step's initialIntegrate (integrate.c:29-59), which kicks with
+                // the same force: half kick, then the drift into the other position buffer
+                vx             = vx + dtforce * (double)rx;
+                vy             = vy + dtforce * (double)ry;
+                vz             = vz + dtforce * (double)rz;
+                xn[3 * si]     = px + dt * vx;
+                xn[3 * si + 1] = py + dt * vy;
+                xn[3 * si + 2] = pz + dt * vz;
+            }
+            v[3 * si]     = vx;
+            v[3 * si + 1] = vy;
+            v[3 * si + 2] = vz;
         }
     }
     if (EV) {
@@ -1513,6 +1536,7 @@ size_t nslots_all() { return ((size_t)C.ncj + C.nghost + 1) * C.N; }
 
 bool ensure_slots(size_t nslot, size_t keep_slots)
 {
+    C.sxn_valid = false;
     grow_n(&C.d_sx, &C.cap_sx, 3 * nslot, 3 * keep_slots);
     grow_n(&C.d_st, &C.cap_st, nslot, keep_slots);
     return !failed();
@@ -1596,7 +1620,7 @@ void freeDeviceClusterGFX950(void)
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
         C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
-        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status };
+        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn };
     for (void* p : ptrs)
         dfree(p);
     if (C.h_status) (void)hipHostFree(C.h_status);
@@ -1750,6 +1774,7 @@ void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy)
 void buildClustersGFX950(Atom* atom)
 {
     if (!ok()) return;
+    C.sxn_valid = false;
     if (C.g.mbins == 0) {
         set_error(E_ARG, "buildClustersGFX950 before setupNeighborClusterGFX950");
         return;
@@ -1818,6 +1843,7 @@ void defineJClustersGFX950(Parameter* param, Atom* atom)
 
 void setupPbcClusterGFX950(Atom* atom, Parameter* param)
 {
+    C.sxn_valid = false;
     if (!ok()) return;
     TimedRegion tr(F_PBC);
     hipStream_t s = stream();
@@ -1860,6 +1886,7 @@ void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate)
     if (!ok() || C.nghost == 0) return;
     TimedRegion tr(F_PBC);
     if (firstUpdate) {
+        C.sxn_valid = false;
         k_cp_ghost_first<<<blocks_for(C.nghost + 1, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost,
             C.N, C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx,
             C.d_st, C.d_cj_bbox);
@@ -1994,14 +2021,17 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
 // The force call; with fuse_final the FP32 full-list kernel also applies finalIntegrate to each
 // atom whose total force it has just written, and returns true (false: not fused, the caller
 // integrates).
-static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final, double* secs);
+// What a force call also does in the same kernel; the caller runs the rest separately.
+enum { FUSED_FINAL = 1, FUSED_INITIAL = 2 };
+static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
+    bool fuse_initial, double* secs);
 
 double computeForceLJClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
 {
     (void)atom;
     (void)stats;
     double secs = 0.0;
-    force_cluster(param, neighbor, false, &secs);
+    force_cluster(param, neighbor, false, false, &secs);
     return secs;
 }
 
@@ -2010,29 +2040,57 @@ double computeForceFinalClusterGFX950(Parameter* param, Atom* atom, Neighbor* ne
 {
     (void)stats;
     double secs = 0.0;
-    if (!force_cluster(param, neighbor, true, &secs)) finalIntegrateClusterGFX950(param, atom);
+    if (!(force_cluster(param, neighbor, true, false, &secs) & FUSED_FINAL))
+        finalIntegrateClusterGFX950(param, atom);
     return secs;
 }
 
-static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final, double* secs)
+double computeForceFinalInitialClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats)
 {
-    if (!ok()) return false;
+    (void)stats;
+    double secs   = 0.0;
+    const int got = force_cluster(param, neighbor, true, true, &secs);
+    if (!(got & FUSED_FINAL)) finalIntegrateClusterGFX950(param, atom);
+    if (!(got & FUSED_INITIAL)) initialIntegrateClusterGFX950(param, atom);
+    return secs;
+}
+
+static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
+    bool fuse_initial, double* secs)
+{
+    if (!ok()) return 0;
     const double t0 = wtime();
-    bool fused      = false;
+    int fused       = 0;
     hipStream_t s   = stream();
     const int nci = C.nci, half = neighbor->half_neigh;
     const size_t nloc = (size_t)nci * C.M;
     if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
+    // the next step's positions go to the second buffer, whose padding, dummy and ghost slots
+    // must match the current one (copied once after each rebuild or upload)
+    double* xn = nullptr;
+    if (fuse_initial && fuse_final && C.precision == 0 && !half) {
+        const size_t n3 = 3 * nslots_all(), cap0 = C.cap_sxn;
+        grow_n(&C.d_sxn, &C.cap_sxn, n3, 0);
+        if (failed()) return 0;
+        if (C.cap_sxn != cap0) C.sxn_valid = false;
+        if (!C.sxn_valid) {
+            MDG_HIP(hipMemcpyAsync(C.d_sxn, C.d_sx, n3 * sizeof(double),
+                hipMemcpyDeviceToDevice, s));
+            C.sxn_valid = true;
+        }
+        xn = C.d_sxn;
+    }
     TimedRegion tr(F_FORCE_CP);
     if (C.precision == 0 &&
         ((size_t)C.ncj + C.nghost + 1 >= (1u << 24) || nslots_all() * 24 >= (1ull << 32))) {
         set_error(E_ARG, "FP32 cluster kernel: %zu slots exceed its 32-bit addressing",
             nslots_all());
-        return false;
+        return 0;
     }
     if (C.precision == 1) {
         grow_n(&C.d_sf64, &C.cap_sf64, 3 * nslots_all(), 0);
-        if (failed()) return false;
+        if (failed()) return 0;
         if (half) fill_f64(C.d_sf64, 0.0, 3 * nslots_all());
         unsigned grid = blocks_for((long long)nloc, BLOCK);
         const int uni = C.uniform ? 1 : 0;
@@ -2057,11 +2115,11 @@ static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         C.forces_are_f32 = false;
     } else {
         grow_n(&C.d_sf32, &C.cap_sf32, 3 * nslots_all(), 0);
-        if (failed()) return false;
+        if (failed()) return 0;
         if (half) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
         unsigned grid = (blocks_for(nci, BLOCK / 64) + 7) & ~7u;  // multiple of 8, see kernel
         // full lists: four i-clusters per wavefront (and the fused final integrate)
-        fused          = fuse_final && !half;
+        fused          = (fuse_final && !half) ? FUSED_FINAL | (xn ? FUSED_INITIAL : 0) : 0;
         double* fuse_v = fused ? C.d_sv : nullptr;
         unsigned grid4 = (blocks_for((nci + 3) / 4, BLOCK / 64) + 7) & ~7u;
 #define MDG_F32(NN, H, E, U, CO)                                                                 \
@@ -2073,7 +2131,7 @@ static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         else                                                                                     \
             k_cp_force_f32_full<NN, E, U, CO><<<grid4, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
                 C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
-                C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce);                          \
+                C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn, param->dt);           \
     } while (0)
 #define MDG_F32_U(NN, H, E)                                                                      \
     do {                                                                                         \
@@ -2103,6 +2161,10 @@ static bool force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         C.forces_are_f32 = true;
     }
     MDG_LAUNCH_CHECK("cp_force");
+    if (fused & FUSED_INITIAL) {
+        std::swap(C.d_sx, C.d_sxn);
+        std::swap(C.cap_sx, C.cap_sxn);
+    }
     *secs = wtime() - t0;
     return fused;
 }
@@ -2190,6 +2252,7 @@ long long uploadClusterArrayGFX950(const char* name, const void* in, long long n
     else if (!strcmp(name, "slot_v")) { dst = C.d_sv; avail = 3LL * C.nci * C.M; }
     if (!dst) return -1;
     if (n > avail || n < 0) return -2;
+    if (dst == C.d_sx) C.sxn_valid = false;
     MDG_HIP(hipMemcpyAsync(dst, in, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream()));
     MDG_HIP(hipStreamSynchronize(stream()));
     return failed() ? -3 : n;

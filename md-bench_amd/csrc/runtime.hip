@@ -64,6 +64,8 @@ struct Pending {
 };
 static bool g_timing = false;
 static unsigned g_timed_mask = ~0u;  // families that record events while timing is on
+static int g_every = 1;              // events on every g_every-th entry of a family's region
+static long long g_seen[F_NFAMILY];
 static double g_ms[F_NFAMILY];
 static long long g_launch[F_NFAMILY];
 static std::vector<Pending> g_pending;
@@ -100,6 +102,7 @@ static void harvest()
 TimedRegion::TimedRegion(Family f) : fam(f), slot(-1)
 {
     if (!g_timing || !(g_timed_mask & (1u << f)) || failed()) return;
+    if (g_seen[f]++ % g_every) return;
     Pending p;
     p.a   = take_event();
     p.b   = take_event();
@@ -291,6 +294,7 @@ void mdgfx950EnableTiming(int on)
         for (int f = 0; f < mdg::F_NFAMILY; f++) {
             mdg::g_ms[f]     = 0;
             mdg::g_launch[f] = 0;
+            mdg::g_seen[f]   = 0;
         }
     }
     if (!on) mdg::harvest();
@@ -310,6 +314,8 @@ int mdgfx950TimeOnly(const char* family)
         }
     return -1;
 }
+
+void mdgfx950TimeEvery(int every) { mdg::g_every = every > 1 ? every : 1; }
 
 int mdgfx950KernelTime(const char* family, double* total_ms, long long* launches)
 {

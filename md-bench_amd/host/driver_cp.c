@@ -194,8 +194,9 @@ int mdbCPRun(mdb_cp* s, int nsteps, double* ksum)
     Parameter* p = &s->param;
     Atom* a      = &s->atom;
     Neighbor* nb = &s->neighbor;
+    int integrated = 0; /* this step's initialIntegrate ran inside the last force call */
     for (int n = 0; n < nsteps; n++) {
-        initialIntegrateClusterGFX950(p, a);
+        if (!integrated) initialIntegrateClusterGFX950(p, a);
         if ((n + 1) % p->reneigh_every) {
             if (!((n + 1) % p->prune_every)) pruneNeighborClusterGFX950(p, a, nb);
             updatePbcClusterGFX950(a, p, false);
@@ -204,7 +205,15 @@ int mdbCPRun(mdb_cp* s, int nsteps, double* ksum)
             updateAtomsPbcClusterGFX950(a, p, false);
             if (reneighbour(s)) return mdgfx950Status();
         }
-        computeForceFinalClusterGFX950(p, a, nb, &s->stats);
+        /* the next step's initialIntegrate rides with this force call unless this step's
+           full-step velocities are read (after the last step, or every step when traced) or
+           the next step prunes before its ghost update: the fused call leaves the ghost slots
+           one step older than the separate calls do, and the prune reads them */
+        const int m      = n + 2; /* the next step, counted as (n + 1) above */
+        const int prunes = (m % p->reneigh_every) && !(m % p->prune_every);
+        integrated       = n + 1 < nsteps && !ksum && !prunes;
+        if (integrated) computeForceFinalInitialClusterGFX950(p, a, nb, &s->stats);
+        else computeForceFinalClusterGFX950(p, a, nb, &s->stats);
         if (ksum) ksum[n] = kineticSumClusterGFX950(p, a);
         if (mdgfx950Status()) return mdgfx950Status();
     }

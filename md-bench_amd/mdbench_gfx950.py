@@ -126,6 +126,8 @@ def lib():
     L.mdgfx950EnableTiming.argtypes = [C.c_int]
     L.mdgfx950TimeOnly.argtypes = [C.c_char_p]
     L.mdgfx950TimeOnly.restype = C.c_int
+    L.mdgfx950TimeEvery.argtypes = [C.c_int]
+    L.mdgfx950TimeEvery.restype = None
     L.mdgfx950KernelTime.argtypes = [C.c_char_p, _PD, C.POINTER(C.c_longlong)]
     # drop-in entry points (opaque struct pointers from the driver)
     for fn in ("computeForceLJGFX950", "computeForceEamGFX950"):
@@ -177,6 +179,8 @@ def lib():
     L.updatePbcClusterGFX950.restype = None
     L.computeForceFinalClusterGFX950.argtypes = [vp, vp, vp, vp]
     L.computeForceFinalClusterGFX950.restype = C.c_double
+    L.computeForceFinalInitialClusterGFX950.argtypes = [vp, vp, vp, vp]
+    L.computeForceFinalInitialClusterGFX950.restype = C.c_double
     L.clusterInfoGFX950.argtypes = [_ip]
     L.setClusterPrecisionGFX950.argtypes = [C.c_int, C.c_int]
     _lib = L

@@ -191,6 +191,32 @@ def test_fused_force_final_equals_separate_calls():
     np.testing.assert_array_equal(fa, fb)
 
 
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape):
+    """The run loop integrates the next step inside the force call
+    (computeForceFinalInitialClusterGFX950, two position buffers) except where a step's full-step
+    velocities are read; a traced run, which reads them every step and so fuses nothing, gives
+    the same trajectory bit for bit, across rebuilds (every 7 steps) and prunes (every 3)."""
+    P = product()
+    M, N = shape
+    kw = dict(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=7, prune_every=3,
+              cluster_m=M, cluster_n=N)
+    out = []
+    for trace in (False, True):
+        sim = P.CPSim(**kw)
+        try:
+            sim.run(10, trace=trace)
+            sim.run(15, trace=trace)
+            live = _live_slots(sim.get("ci_natoms"), M)
+            # (forces of empty and ghost slots are never written)
+            out.append([sim.get("slot_x"), sim.get("slot_v"),
+                        sim.get("slot_f").reshape(-1, 3)[live], sim.get("neighbors")])
+        finally:
+            sim.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
 def _thermalised_atoms(nx, steps=60):
     """A liquid-like state from the Verlet-list oracle (FP64), wrapped into the box."""
     P = product()
