@@ -127,8 +127,9 @@ def bench_vl(args, mod):
         elapsed = t1 - t0
         # Algorithmic bytes of one full-list FP64 force launch (MD-Bench's data-volume model,
         # src/verletlist/stats.c:56-59): per atom x_i (24 B) + f_i (24 B) + numneigh (4 B),
-        # per neighbour index (4 B) + x_j (24 B).
-        bytes_launch = nlocal * 52.0 + sum_nn * 28.0
+        # per neighbour index (4 B) + x_j (24 B); the run loop's launches also integrate (per
+        # atom v read and written, the new position written: 72 B).
+        bytes_launch = nlocal * (52.0 + 72.0) + sum_nn * 28.0
         avg_ms = ms.value / max(nl.value, 1)
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         res = {
@@ -288,8 +289,9 @@ def bench_eam(args, mod, nx=32):
         # Algorithmic bytes of one EAM force call (two passes over the list; spline tables stay in
         # cache and are not counted).  Density: per atom x_i, count, fp_i write (36 B), per listed
         # neighbour index + x_j (28 B).  Force: per atom x_i, fp_i, count, f_i write (60 B), per
-        # listed neighbour index + x_j + fp_j (36 B).
-        bytes_launch = nlocal * (36.0 + 60.0) + sum_nn * (28.0 + 36.0)
+        # listed neighbour index + x_j + fp_j (36 B).  The run loop's force pass also integrates
+        # (per atom v read and written, the new position written: 72 B).
+        bytes_launch = nlocal * (36.0 + 60.0 + 72.0) + sum_nn * (28.0 + 36.0)
         avg_ms = ms.value / max(nl.value, 1)
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         res = {

@@ -89,6 +89,16 @@ void updateAtomsPbcGFX950(Atom* atom, Parameter* param, bool doReneighbor);
 double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
 /* replaces computeForceEam (src/verletlist/force_eam.c:19-228); tables from setEamTablesGFX950 */
 double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats);
+/* No reference counterpart: computeForce (LJ or EAM by param->force_field), finalIntegrate and
+   the NEXT step's initialIntegrate (src/verletlist/main.c loop) in one call.  The fast full-list
+   kernels kick each atom twice with the force they have just written and drift it into a second
+   position buffer, which becomes atom->d_atom.x; the same FP64 operations in the same order as
+   the three calls.  Ghost positions are stale until updatePbcGFX950, as after initialIntegrate.
+   Velocities are left at the next half step: callers reading full-step velocities after a step
+   (thermo, the last step) call the three separately there.  Strict kernels and half lists run
+   the three calls in order. */
+double computeForceFinalInitialGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats);
 /* uploads one pair table in the reference's 7-coefficient layout (src/common/eam.h:20-29,
    eam_utils.c:222-284); replaces the global `Eam eam` the reference kernel reads */
 void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double* rhor_spline,

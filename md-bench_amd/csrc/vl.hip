@@ -43,6 +43,10 @@ struct State {
     double prd[3]{}, cutneigh = 0;
     // atoms
     size_t cap_all = 0, cap_ghost = 0, cap_v = 0;
+    // second position buffer: a force call that also integrates the next step writes the new
+    // local positions there (its other lanes still read the current ones) and the two swap
+    double* d_xn  = nullptr;
+    size_t cap_xn = 0;
     int ntypes = 0;
     bool uniform = true;  // every type pair has the same LJ parameters
     Tables tab{};
@@ -392,13 +396,38 @@ __global__ void __launch_bounds__(BLOCK) k_build_neighbor(int nlocal, const doub
     }
 }
 
+// The run loop's integration inside a full-list force kernel: with v set, the lane that writes
+// an atom's total force also applies finalIntegrate (integrate.c:61-86) and the next step's
+// initialIntegrate (integrate.c:29-59) with it, the same FP64 operations in the same order as
+// k_final_integrate then k_initial_integrate, and writes the new position to xn.
+struct Fuse {
+    double* v;
+    double* xn;
+    double dt, dtforce;
+};
+
+// v0 (loaded at the kernel's start, so its latency hides behind the neighbour loop) and x0 are
+// the atom's velocity and position.
+__device__ inline void fused_integrate(const Fuse& fu, int i, const double* v0, const double* x0,
+    double fx, double fy, double fz)
+{
+    const double fr[3] = { fx, fy, fz };
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double vv         = v0[d] + fu.dtforce * fr[d];
+        vv                = vv + fu.dtforce * fr[d];
+        fu.v[3 * i + d]   = vv;
+        fu.xn[3 * i + d]  = x0[d] + fu.dt * vv;
+    }
+}
+
 // ---------------------------------------------------------------- LJ (force_lj.c:17-206)
 // One atom per LPA lanes.  STRICT: LPA == 1, reference order, no fma.
 template <int LPA, bool STRICT, bool HALF, bool EV>
 __global__ void __launch_bounds__(BLOCK) k_force_lj(int nlocal, const double* __restrict__ x,
     const int* __restrict__ type, int ntypes, Tables tab, int uniform,
     const int* __restrict__ neighbors, const int* __restrict__ numneigh, double* __restrict__ f,
-    double* __restrict__ ev)
+    double* __restrict__ ev, Fuse fu)
 {
     __shared__ double s_tab[3][MAXTYPES2];
     __shared__ double s_red[16];
@@ -414,8 +443,15 @@ __global__ void __launch_bounds__(BLOCK) k_force_lj(int nlocal, const double* __
     const int lane = gid % LPA;
     const bool act = i < nlocal;
     double fix = 0, fiy = 0, fiz = 0, e = 0, w = 0;
+    double v0[3] = {}, x0[3] = {};
+    if (!HALF && fu.v && act && lane == 0)
+        for (int d = 0; d < 3; d++)
+            v0[d] = fu.v[3 * i + d];
     if (act) {
         const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
+        x0[0] = xt;
+        x0[1] = yt;
+        x0[2] = zt;
         const int ti    = type[i];
         const int nn    = numneigh[i];
         double eps0 = s_tab[0][0], sig0 = s_tab[1][0], cut0 = s_tab[2][0];
@@ -474,6 +510,7 @@ __global__ void __launch_bounds__(BLOCK) k_force_lj(int nlocal, const double* __
             f[3 * i]     = fix;
             f[3 * i + 1] = fiy;
             f[3 * i + 2] = fiz;
+            if (fu.v) fused_integrate(fu, i, v0, x0, fix, fiy, fiz);
         }
     }
     if (EV) {
@@ -553,7 +590,7 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
     const int* __restrict__ neighbors, const int* __restrict__ numneigh, double cutforcesq,
     const double* __restrict__ rs, const double* __restrict__ zs, int nr, double rdr,
     const double* __restrict__ fp, double* __restrict__ f, double* __restrict__ ev,
-    double* __restrict__ eatom)
+    double* __restrict__ eatom, Fuse fu)
 {
     __shared__ double s_red[16];
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,8 +598,15 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
     const int lane = gid % LPA;
     const bool act = i < nlocal;
     double fix = 0, fiy = 0, fiz = 0, e = 0, w = 0;
+    double v0[3] = {}, x0[3] = {};
+    if (fu.v && act && lane == 0)
+        for (int d = 0; d < 3; d++)
+            v0[d] = fu.v[3 * i + d];
     if (act) {
         const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
+        x0[0] = xt;
+        x0[1] = yt;
+        x0[2] = zt;
         const double fpi = fp[i];
         const int nn     = numneigh[i];
         for (int k = lane; k < nn; k += LPA) {
@@ -616,6 +660,7 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
         f[3 * i + 1] = fiy;
         f[3 * i + 2] = fiz;
         if (EV && eatom) eatom[i] += ea;
+        if (fu.v) fused_integrate(fu, i, v0, x0, fix, fiy, fiz);
     }
     if (EV) {
         double be = block_sum(e, s_red);
@@ -766,6 +811,7 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
         neighbor->d_neighbor.neighbors = nullptr;
         neighbor->d_neighbor.numneigh  = nullptr;
     }
+    dfree(S.d_xn);
     dfree(S.d_stencil);
     dfree(S.d_bin_of);
     dfree(S.d_bincount);
@@ -1053,10 +1099,9 @@ void updateAtomsPbcGFX950(Atom* atom, Parameter* param, bool doReneighbor)
     MDG_LAUNCH_CHECK("wrap");
 }
 
-double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
+static double force_lj(Parameter* param, Atom* atom, Neighbor* neighbor, Fuse fu)
 {
     (void)param;
-    (void)stats;
     if (failed() || !S.ready) return 0.0;
     const double t0 = wtime();
     const int nlocal = atom->Nlocal;
@@ -1073,7 +1118,7 @@ double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, St
 #define MDG_LJ(L, ST, H, E)                                                                      \
     k_force_lj<L, ST, H, E><<<grid, BLOCK, 0, s>>>(nlocal, atom->d_atom.x, atom->d_atom.type,    \
         S.ntypes, S.tab, S.uniform ? 1 : 0, neighbor->d_neighbor.neighbors,                      \
-        neighbor->d_neighbor.numneigh, atom->d_atom.fx, S.d_ev)
+        neighbor->d_neighbor.numneigh, atom->d_atom.fx, S.d_ev, fu)
         if (strict) {
             if (half) {
                 if (S.ev_on) MDG_LJ(1, true, true, true);
@@ -1097,6 +1142,12 @@ double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, St
     return wtime() - t0;
 }
 
+double computeForceLJGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
+{
+    (void)stats;
+    return force_lj(param, atom, neighbor, Fuse{ nullptr, nullptr, 0.0, 0.0 });
+}
+
 void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double* rhor_spline,
     const double* frho_spline, const double* z2r_spline, int nr_tot, int nrho_tot)
 {
@@ -1118,9 +1169,8 @@ void setEamTablesGFX950(int nr, int nrho, double rdr, double rdrho, const double
     S.rdrho = rdrho;
 }
 
-double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
+static double force_eam(Parameter* param, Atom* atom, Neighbor* neighbor, Fuse fu)
 {
-    (void)stats;
     if (failed() || !S.ready) return 0.0;
     if (!S.d_rhor) {
         set_error(MDG_ERR_ARG, "computeForceEamGFX950 before setEamTablesGFX950");
@@ -1153,7 +1203,7 @@ double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, S
         launch_ghost_copy(S.d_fp, atom);                                                         \
         k_eam_force<L, ST, E><<<grid, BLOCK, 0, s>>>(nlocal, atom->d_atom.x,                     \
             neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
-            S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, S.d_ev, eatom);                       \
+            S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, S.d_ev, eatom, fu);                   \
     } while (0)
     if (strict) {
         if (ev) MDG_EAM(1, true, true);
@@ -1165,6 +1215,35 @@ double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, S
 #undef MDG_EAM
     MDG_LAUNCH_CHECK("force_eam");
     return wtime() - t0;
+}
+
+double computeForceEamGFX950(Parameter* param, Atom* atom, Neighbor* neighbor, Stats* stats)
+{
+    (void)stats;
+    return force_eam(param, atom, neighbor, Fuse{ nullptr, nullptr, 0.0, 0.0 });
+}
+
+double computeForceFinalInitialGFX950(Parameter* param, Atom* atom, Neighbor* neighbor,
+    Stats* stats)
+{
+    if (failed() || !S.ready) return 0.0;
+    const bool eam = param->force_field == FF_EAM;
+    if (strict_mode() || (!eam && neighbor->half_neigh)) {
+        // the strict kernels and half lists (atomic forces) run the three calls in order
+        const double t = eam ? computeForceEamGFX950(param, atom, neighbor, stats)
+                             : computeForceLJGFX950(param, atom, neighbor, stats);
+        finalIntegrateGFX950(false, param, atom);
+        initialIntegrateGFX950(false, param, atom);
+        return t;
+    }
+    grow_n(&S.d_xn, &S.cap_xn, 3 * S.cap_all, 0);
+    if (failed()) return 0.0;
+    const Fuse fu{ atom->d_atom.vx, S.d_xn, param->dt, param->dtforce };
+    const double t = eam ? force_eam(param, atom, neighbor, fu) : force_lj(param, atom, neighbor, fu);
+    // the new local positions become the current ones; ghost positions follow at updatePbc
+    std::swap(atom->d_atom.x, S.d_xn);
+    S.cap_xn = 3 * S.cap_all;
+    return t;
 }
 
 void requestEamAtomEnergyGFX950(int on)

@@ -231,9 +231,10 @@ int mdbVLRun(mdb_vl* s, int nsteps, double* ksum)
     Parameter* p = &s->param;
     Atom* a      = &s->atom;
     Neighbor* nb = &s->neighbor;
+    int integrated = 0; /* this step's initialIntegrate ran inside the last force call */
     for (int n = 0; n < nsteps; n++) {
         int reneigh = (n + 1) % p->reneigh_every == 0;
-        initialIntegrateGFX950(reneigh, p, a);
+        if (!integrated) initialIntegrateGFX950(reneigh, p, a);
         if (reneigh) {
             updateAtomsPbcGFX950(a, p, true);
             setupPbcGFX950(a, p);
@@ -241,6 +242,16 @@ int mdbVLRun(mdb_vl* s, int nsteps, double* ksum)
             buildNeighborGFX950(a, nb);
         } else {
             updatePbcGFX950(a, p, false);
+        }
+        /* plain LJ/EAM steps carry the next step's initialIntegrate in the force call, unless
+           this step's full-step velocities are read (after the last step, or every step when
+           traced) or the surrogate runs (its judge reads this step's positions) */
+        integrated = n + 1 < nsteps && !ksum && !s->sur_on;
+        if (integrated) {
+            computeForceFinalInitialGFX950(p, a, nb, &s->stats);
+            s->steps_done++;
+            if (mdgfx950Status()) return mdgfx950Status();
+            continue;
         }
         compute_force(s);
         s->steps_done++;

@@ -181,6 +181,8 @@ def lib():
     L.computeForceFinalClusterGFX950.restype = C.c_double
     L.computeForceFinalInitialClusterGFX950.argtypes = [vp, vp, vp, vp]
     L.computeForceFinalInitialClusterGFX950.restype = C.c_double
+    L.computeForceFinalInitialGFX950.argtypes = [vp, vp, vp, vp]
+    L.computeForceFinalInitialGFX950.restype = C.c_double
     L.clusterInfoGFX950.argtypes = [_ip]
     L.setClusterPrecisionGFX950.argtypes = [C.c_int, C.c_int]
     _lib = L

@@ -160,6 +160,31 @@ def test_default_testcase_regression(strict):
         sim.close()
 
 
+@pytest.mark.parametrize("ff", [0, 1], ids=["lj", "eam"])
+def test_fused_integration_equals_unfused_run(ff):
+    """The run loop integrates the next step inside the force call (computeForceFinalInitial-
+    GFX950, two position buffers) except where a step's full-step velocities are read; a traced
+    run, which reads them every step and so fuses nothing, gives the same state bit for bit,
+    across rebuilds every 7 steps.  (Half lists run the separate calls; their atomic force sums
+    have no fixed order, so two half-list runs agree to rounding only.)"""
+    P = product()
+    kw = dict(nx=6, ny=6, nz=6, ntimes=0, reneigh_every=7)
+    if ff:
+        kw.update(force_field=1, lattice_a=3.615, temp=1600.0)
+    out = []
+    for trace in (False, True):
+        sim = P.VLSim(**kw)
+        try:
+            sim.run(10, trace=trace)
+            sim.run(15, trace=trace)
+            st = sim.state()
+            out.append([st[k] for k in ("x", "v", "f", "numneigh", "neighbors")])
+        finally:
+            sim.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("strict", [1, 0])
 def test_eam_atom_energy_matches_oracle(strict):
     """Per-atom EAM energies (the learned potential's training labels) on a thermalised 6^3 Cu
