@@ -11,8 +11,9 @@
 //                 16 columns).  A layer's accumulator layout is the next layer's B operand once
 //                 the hidden features are numbered to match, so activations never leave
 //                 registers.  The epilogue turns dE/dD into 60 force coefficients per atom.
-//   k_sur_force   LPA lanes per atom: F_k = sum_j (dE_k/dx_j - dE_j/dx_k), the j side from the
-//                 neighbour's coefficient row (a ghost reads its owner's row).
+//   k_sur_force   LPA lanes per atom, one pair per lane: F_k = sum_j (dE_k/dx_j - dE_j/dx_k),
+//                 the j side from the neighbour's coefficient row (a ghost reads its owner's
+//                 row), the atom's own row staged in LDS.
 // Training adds the force adjoint (k_sur_adjoint), the exact FP32 weight gradient of the loss
 // (k_sur_wgrad: forward mode along the adjoint, then reverse mode, block partials reduced in a
 // fixed order) and Adam (k_sur_adam).
@@ -530,64 +531,144 @@ __device__ inline void load_block(const __half* __restrict__ p, float* c)
     }
 }
 
+// Lane per pair: the atom's LPA lanes take its in-cutoff pairs in turn (lane q: entries q,
+// q + LPA, ...) and each evaluates all four coefficient blocks of its pair, so the pair's
+// geometry, the neighbour lookup, the core term and the force accumulation run once per pair.
+// The atom's own coefficients sit in LDS (rows padded to 68 floats: the 16 atoms of a wavefront
+// read different banks).  Block b: radial descriptors 6b .. 6b+5 and angular channel b.
+constexpr int CI_STRIDE = 68;
 __global__ void __launch_bounds__(BLOCK) k_sur_force(int nlocal, int i0, int n,
     const double* __restrict__ x, const int* __restrict__ border_map, const int* __restrict__ cl,
     const int* __restrict__ ncl, const __half* __restrict__ coef, float rcore, float kcore,
     double* __restrict__ f)
 {
+    __shared__ float s_ci[BLOCK / LPA][CI_STRIDE];
     const int gid  = blockIdx.x * blockDim.x + threadIdx.x;
     const int a    = gid / LPA;
     const int q    = gid % LPA;
+    const int la   = threadIdx.x / LPA;
     const bool act = a < n;
     const int i    = i0 + (act ? a : 0);
-    float ci[16];
-    load_block(coef + (size_t)(act ? a : 0) * NCOEF + 16 * q, ci);
-    const float amu = A_MU0 + q * A_DMU;
-    const float k0n = 2.0f * R_ETA * (R_MU0 + RPL * q * R_DMU);  // k_n = -2 eta r + k0n + n dk
+    {
+        float c[16];
+        load_block(coef + (size_t)(act ? a : 0) * NCOEF + 16 * q, c);
+#pragma unroll
+        for (int m = 0; m < 16; m++)
+            s_ci[la][16 * q + m] = c[m];
+    }
+    __syncthreads();
+    const float* ci = s_ci[la];
+    const float dk  = 2.0f * R_ETA * R_DMU;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     if (act) {
         const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
         const int nn = ncl[a];
-        for (int k = 0; k < nn; k++) {
+        for (int k = q; k < nn; k += LPA) {
             const int j = cl[(size_t)k * n + a];
             Pair p;
             if (!pair_geom(x, xi, yi, zi, j, &p)) continue;
             const int rj = coef_row(j, nlocal, border_map) - i0;
             if (rj < 0 || rj >= n) continue;  // owner outside the range: the caller's error
-            float cj[16];
-            load_block(coef + (size_t)rj * NCOEF + 16 * q, cj);
-            float g[RPL];
-            gaussians6(p.r, q, g);
-            // radial: S = sum_n (c_i + c_j)_n g_n (fc k_n + dfc)
-            float kn       = fmaf(-2.0f * R_ETA, p.r, k0n);  // -2 eta (r - mu_{6q})
-            const float dk = 2.0f * R_ETA * R_DMU;
-            float S        = 0.f;
+            // sum over blocks of the coefficient of u and of the vector terms.  Every product
+            // with a neighbour coefficient is an explicit fma with the FP16 value as a factor
+            // (one v_fma_mix_f32 each, no conversion).
+            float su = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+            auto block = [&](int b, const float* g) {
+                const __half2* hj = reinterpret_cast<const __half2*>(coef + (size_t)rj * NCOEF + 16 * b);
+                __half2 h[8];
+                {
+                    const uint4* v = reinterpret_cast<const uint4*>(hj);
+                    const uint4 w0 = v[0], w1 = v[1];
+                    const unsigned u[8] = { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w };
 #pragma unroll
-            for (int m = 0; m < RPL; m++) {
-                S = fmaf((ci[m] + cj[m]) * g[m], fmaf(p.fc, kn, p.dfc), S);
-                kn += dk;
+                    for (int k2 = 0; k2 < 8; k2++)
+                        h[k2] = *reinterpret_cast<const __half2*>(&u[k2]);
+                }
+                auto cj = [&](int m) {
+                    return (m & 1) ? __high2float(h[m >> 1]) : __low2float(h[m >> 1]);
+                };
+                const float* cb = ci + 16 * b;
+                // radial: sum_n (c_i + c_j)_n g_n (fc k_n + dfc), k_n = -2 eta (r - mu_n)
+                float tk          = fmaf(p.fc,
+                             fmaf(-2.0f * R_ETA, p.r, 2.0f * R_ETA * (R_MU0 + RPL * b * R_DMU)), p.dfc);
+                const float tstep = p.fc * dk;
+#pragma unroll
+                for (int m = 0; m < RPL; m++) {
+                    const float gt = g[m] * tk;
+                    su             = fmaf(cb[m], gt, su);
+                    su             = fmaf(cj(m), gt, su);
+                    tk += tstep;
+                }
+                // angular channel b, both atoms at once: a = a_i - a_j, S = S_i + S_j
+                const float dr = p.r - (A_MU0 + b * A_DMU);
+                const float ex = __expf(-A_ETA * dr * dr);
+                const float dw = ex * fmaf(-2.0f * A_ETA * dr, p.fc, p.dfc);
+                const float wr = ex * p.fc * p.ir;
+                const float w2 = 2.0f * wr;
+                float ua       = fmaf(p.uz, cb[8], fmaf(p.uy, cb[7], p.ux * cb[6]));
+                ua             = fmaf(-p.uz, cj(8), fmaf(-p.uy, cj(7), fmaf(-p.ux, cj(6), ua)));
+                // M = S u, S symmetric: (xx, yy, zz, xy, xz, yz) at 9..14
+                float mx = fmaf(cb[9], p.ux, fmaf(cb[12], p.uy, cb[13] * p.uz));
+                mx       = fmaf(cj(9), p.ux, fmaf(cj(12), p.uy, fmaf(cj(13), p.uz, mx)));
+                float my = fmaf(cb[12], p.ux, fmaf(cb[10], p.uy, cb[14] * p.uz));
+                my       = fmaf(cj(12), p.ux, fmaf(cj(10), p.uy, fmaf(cj(14), p.uz, my)));
+                float mz = fmaf(cb[13], p.ux, fmaf(cb[14], p.uy, cb[11] * p.uz));
+                mz       = fmaf(cj(13), p.ux, fmaf(cj(14), p.uy, fmaf(cj(11), p.uz, mz)));
+                const float uMu = fmaf(p.ux, mx, fmaf(p.uy, my, p.uz * mz));
+                su              = fmaf(dw, ua + uMu, su);
+                su              = fmaf(-wr, ua, su);
+                su              = fmaf(-w2, uMu, su);
+                vx = fmaf(w2, mx, fmaf(-wr, cj(6), fmaf(wr, cb[6], vx)));
+                vy = fmaf(w2, my, fmaf(-wr, cj(7), fmaf(wr, cb[7], vy)));
+                vz = fmaf(w2, mz, fmaf(-wr, cj(8), fmaf(wr, cb[8], vz)));
+            };
+            // the 24 Gaussians by one recurrence from the middle one (as gaussians24): blocks 2
+            // and 3 upwards, then 1 and 0 downwards, six values live at a time
+            {
+                const float t  = p.r - (R_MU0 + RMID * R_DMU);
+                const float e2 = R_ETA * R_DMU * R_DMU;
+                const float c  = __expf(-2.0f * e2);
+                float up       = __expf(2.0f * R_ETA * R_DMU * t - e2);
+                float dn       = __expf(-2.0f * R_ETA * R_DMU * t - e2);
+                const float gm = __expf(-R_ETA * t * t);
+                float g[RPL];
+                g[0] = gm;
+#pragma unroll
+                for (int m = 1; m < RPL; m++) {
+                    g[m] = g[m - 1] * up;
+                    up *= c;
+                }
+                block(2, g);
+                float gl = g[RPL - 1];
+#pragma unroll
+                for (int m = 0; m < RPL; m++) {
+                    gl   = gl * up;
+                    up *= c;
+                    g[m] = gl;
+                }
+                block(3, g);
+                gl = gm;
+#pragma unroll
+                for (int m = RPL - 1; m >= 0; m--) {
+                    gl   = gl * dn;
+                    dn *= c;
+                    g[m] = gl;
+                }
+                block(1, g);
+                gl = g[0];
+#pragma unroll
+                for (int m = RPL - 1; m >= 0; m--) {
+                    gl   = gl * dn;
+                    dn *= c;
+                    g[m] = gl;
+                }
+                block(0, g);
             }
-            // angular channel q, both atoms at once
-            const float dr  = p.r - amu;
-            const float ex  = __expf(-A_ETA * dr * dr);
-            const float w   = ex * p.fc;
-            const float dw  = ex * fmaf(-2.0f * A_ETA * dr, p.fc, p.dfc);
-            const float wr  = w * p.ir;
-            const float dax = ci[6] - cj[6], day = ci[7] - cj[7], daz = ci[8] - cj[8];
-            const float sxx = ci[9] + cj[9], syy = ci[10] + cj[10], szz = ci[11] + cj[11];
-            const float sxy = ci[12] + cj[12], sxz = ci[13] + cj[13], syz = ci[14] + cj[14];
-            const float ua  = p.ux * dax + p.uy * day + p.uz * daz;
-            const float mx  = sxx * p.ux + sxy * p.uy + sxz * p.uz;
-            const float my  = sxy * p.ux + syy * p.uy + syz * p.uz;
-            const float mz  = sxz * p.ux + syz * p.uy + szz * p.uz;
-            const float uMu = p.ux * mx + p.uy * my + p.uz * mz;
-            float sa        = S + dw * (ua + uMu);
-            // the core pair term, once per pair (lane 0): F_i += U'(r) u, U' = -3 k (rc - r)^2
-            if (q == 0 && p.r < rcore) sa -= 3.0f * kcore * (rcore - p.r) * (rcore - p.r);
-            const float w2  = 2.0f * wr;
-            fx += sa * p.ux + wr * (dax - ua * p.ux) + w2 * (mx - uMu * p.ux);
-            fy += sa * p.uy + wr * (day - ua * p.uy) + w2 * (my - uMu * p.uy);
-            fz += sa * p.uz + wr * (daz - ua * p.uz) + w2 * (mz - uMu * p.uz);
+            // the core pair term: F_i += U'(r) u, U' = -3 k (rc - r)^2
+            if (p.r < rcore) su -= 3.0f * kcore * (rcore - p.r) * (rcore - p.r);
+            fx += su * p.ux + vx;
+            fy += su * p.uy + vy;
+            fz += su * p.uz + vz;
         }
     }
     fx = gsum<LPA>(fx);
