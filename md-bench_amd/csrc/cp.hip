@@ -16,6 +16,7 @@
 //                   in FP32 with a Newton-refined v_rcp_f32.  The
 //                   full list is atomic-free; the half list reduces j-forces over the four
 //                   i-lanes with cross-lane shuffles and issues one atomic per j-atom.
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1244,8 +1245,8 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
 // with all four i-atoms.  Every j position is loaded by one lane only and serves four pairs, and
 // each lane keeps the forces of its group's four i-atoms; the 16 lanes of a group reduce them at
 // the end.  The four lists of a wavefront run to the longest; shorter ones read the dummy
-// j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and arithmetic
-// are those of k_cp_force_f32.  The one-type variant is held to 80 VGPRs, six wavefronts per
+// j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and the
+// arithmetic after the displacement are those of k_cp_force_f32.  The one-type variant is held to 80 VGPRs, six wavefronts per
 // SIMD, without spills; the type-table, energy and compensated variants keep their register
 // count (held to 80 they spill inside the loop; held to 96 the type-table one gains nothing).
 template <int N, bool EV, bool UNIFORM, bool COMP>
@@ -1253,9 +1254,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((UNI
 k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
     const int* __restrict__ nb, const int* __restrict__ numneigh,
-    const int* __restrict__ ci_natoms, int maxneighs, float* __restrict__ f,
-    double* __restrict__ ev, double* __restrict__ v, double dtforce, double* __restrict__ xn,
-    double dt)
+    const int* __restrict__ ci_natoms, const double* __restrict__ ci_bbox, int maxneighs,
+    float* __restrict__ f, double* __restrict__ ev, double* __restrict__ v, double dtforce,
+    double* __restrict__ xn, double dt)
 {
     constexpr int CPG = 16 / N;  // j-clusters per group per iteration
     __shared__ float s_tab[4][MAXT2];
@@ -1280,13 +1281,32 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     const unsigned jj24 = 24u * (unsigned)jj;
     const int ci    = wv * 4 + grp;
     const bool live = ci < nci;
-    double xi[4][3];
+    // One-type variants (SHIFT): positions relative to the centre of the i-cluster's bounding
+    // box (as of the list build; an empty cluster's box is inverted and its centre taken at 0),
+    // the i-atoms once per launch and a j-atom once per cluster pair, each an FP64 difference
+    // rounded to FP32; a pair's displacement is the FP32 difference of the two: three
+    // subtractions per pair instead of three FP64 subtractions and three conversions, ~10 %
+    // fewer vector-ALU cycles.  The two extra roundings raise the 1M-atom LJ force error from
+    // 6.0e-7 to 8.1e-7 (measured), and the Kob-Andersen mixture's from 7.3e-7 to 1.04e-6, over
+    // the 1e-6 bound: the type-table variants keep the rounded FP64 displacement.
+    constexpr bool SHIFT = UNIFORM;
+    using PT             = typename std::conditional<SHIFT, float, double>::type;
+    double ref[3]        = { 0.0, 0.0, 0.0 };
+    if (SHIFT) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double lo = live ? ci_bbox[6 * ci + 2 * d] : 0.0;
+            const double hi = live ? ci_bbox[6 * ci + 2 * d + 1] : 0.0;
+            ref[d]          = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+        }
+    }
+    PT xi[4][3];
     int ti[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
 #pragma unroll
         for (int d = 0; d < 3; d++)
-            xi[a][d] = live ? sx[3 * (ci * 4 + a) + d] : 0.0;
+            xi[a][d] = (PT)((live ? sx[3 * (ci * 4 + a) + d] : 0.0) - ref[d]);
         ti[a] = (UNIFORM || !live) ? 0 : st[ci * 4 + a];
     }
     const int nn = live ? numneigh[ci] : 0;
@@ -1306,13 +1326,22 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         int sj;
     };
     auto use = [&](const JL& J) {
-        const int tj = UNIFORM ? 0 : st[J.sj];
+        const int tj   = UNIFORM ? 0 : st[J.sj];
+        const float jx = SHIFT ? (float)(J.x - ref[0]) : 0.f,
+                    jy = SHIFT ? (float)(J.y - ref[1]) : 0.f,
+                    jz = SHIFT ? (float)(J.z - ref[2]) : 0.f;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            const double dxd = xi[a][0] - J.x;
-            const double dyd = xi[a][1] - J.y;
-            const double dzd = xi[a][2] - J.z;
-            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            float dx, dy, dz;
+            if (SHIFT) {
+                dx = (float)xi[a][0] - jx;
+                dy = (float)xi[a][1] - jy;
+                dz = (float)xi[a][2] - jz;
+            } else {
+                dx = (float)((double)xi[a][0] - J.x);
+                dy = (float)((double)xi[a][1] - J.y);
+                dz = (float)((double)xi[a][2] - J.z);
+            }
             const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
             double cut2 = cut2u;
@@ -1329,6 +1358,23 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             bool in          = rsq < clo;
             const bool below = rsq < chi;
             if (__builtin_expect((__ballot(below) & ~__ballot(in)) != 0, 0)) {
+                // FP64 re-check from the FP64 positions; with SHIFT the i-atom's is read again
+                // (its slot from the wavefront index and the lane id, the latter computed here
+                // by volatile asm, which keeps the compiler from hoisting the address out of the
+                // loop and holding it in a register or spill slot for this rare branch)
+                double pxi = (double)xi[a][0], pyi = (double)xi[a][1], pzi = (double)xi[a][2];
+                if (SHIFT) {
+                    int lr;
+                    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
+                                 : "=v"(lr));
+                    const double* pi = sx + 3 * ((wv * 4 + (lr >> 4)) * 4 + a);
+                    pxi              = pi[0];
+                    pyi              = pi[1];
+                    pzi              = pi[2];
+                }
+                const double dxd = pxi - J.x;
+                const double dyd = pyi - J.y;
+                const double dzd = pzi - J.z;
                 const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
                 if (below && !in) in = rsqd < cut2;
             }
@@ -1418,11 +1464,13 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                 rx = COMP ? fx[a] + cx[a] : fx[a];
                 ry = COMP ? fy[a] + cy[a] : fy[a];
                 rz = COMP ? fz[a] + cz[a] : fz[a];
-                px = xi[a][0];
-                py = xi[a][1];
-                pz = xi[a][2];
             }
         const int si  = ci_e * 4 + sub;
+        if (xn) {
+            px = sx[3 * si];
+            py = sx[3 * si + 1];
+            pz = sx[3 * si + 2];
+        }
         f[3 * si]     = rx;
         f[3 * si + 1] = ry;
         f[3 * si + 2] = rz;
@@ -2131,7 +2179,8 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         else                                                                                     \
             k_cp_force_f32_full<NN, E, U, CO><<<grid4, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
                 C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
-                C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn, param->dt);           \
+                C.d_ci_bbox, C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn,          \
+                param->dt);                                                                      \
     } while (0)
 #define MDG_F32_U(NN, H, E)                                                                      \
     do {                                                                                         \
