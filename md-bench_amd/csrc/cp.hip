@@ -1325,7 +1325,9 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         double x, y, z;
         int sj;
     };
-    auto use = [&](const JL& J) {
+    // SELF: the cluster pair may hold pairs at distance 0 (see the list loop below)
+    auto use = [&](const JL& J, auto self) {
+        constexpr bool SELF = decltype(self)::value;
         const int tj   = UNIFORM ? 0 : st[J.sj];
         const float jx = SHIFT ? (float)(J.x - ref[0]) : 0.f,
                     jy = SHIFT ? (float)(J.y - ref[1]) : 0.f,
@@ -1378,7 +1380,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                 const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
                 if (below && !in) in = rsqd < cut2;
             }
-            in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
+            if (SELF || EV) in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
             // v_rcp_f32 alone: with a Newton step the 1M-atom force error is 5.7e-7, without it
             // 6.0e-7 (measured); the step cost two of the ~25 instructions of a pair
             const float r2i = __builtin_amdgcn_rcpf(rsq);
@@ -1405,6 +1407,14 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     // handed out with cross-lane reads inside the group; entries past a list's end read as the
     // dummy j-cluster.  Positions of the next iteration load while the current ones are used,
     // and every load in the loop is unconditional (see k_cp_force_f32).
+    // The list builder puts the entries that need the diagonal mask first, and in a full list
+    // the only such entry is the j-cluster holding the i-cluster itself: of a cluster with atoms
+    // it is entry 0, and an empty cluster has no entries.  Only the first iteration can meet a
+    // pair at distance 0 and only it tests for one; elsewhere the test is dropped (one compare
+    // and one scalar op per pair).  Without the test a padding i-slot whose FP32 offset equals
+    // that of a padding j-slot (sentinels near 1e12 round to multiples of 2^16) gets a NaN
+    // force, in accumulators of its own that are never written out; the energy variant, whose
+    // sum would take the NaN, keeps the test everywhere.
     for (int c0 = 0; c0 < nmax; c0 += 32) {
         const int e0   = c0 + sub < nn ? lst[c0 + sub] : dummy_cj;
         const int e1   = c0 + 16 + sub < nn ? lst[c0 + 16 + sub] : dummy_cj;
@@ -1421,13 +1431,20 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         };
         JL A, B;
         fetch(c0, A);
-        for (int k0 = c0; k0 < cend;) {
+        int k0 = c0;
+        if (c0 == 0) {
+            fetch(CPG, B);
+            use(A, std::true_type{});
+            A  = B;
+            k0 = CPG;
+        }
+        while (k0 < cend) {
             fetch(k0 + CPG, B);
-            use(A);
+            use(A, std::false_type{});
             k0 += CPG;
             if (k0 >= cend) break;
             fetch(k0 + CPG, A);
-            use(B);
+            use(B, std::false_type{});
             k0 += CPG;
         }
     }
