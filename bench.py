@@ -13,6 +13,8 @@ Workloads (seeded fcc lattices, synthetic velocities):
              rho* 0.8442, r_c 2.5, skin 0.3, T 1.44, dt 0.005, reneighbour every 20 steps.
   cp-lj-64-4x8, cp-lj-64-half: the same with j-clusters of 8 / a half list.
   vl-lj-64   Verlet-list LJ, FP64, full list, same lattice.
+             vl-lj-64-sort100: the same with the atoms re-sorted by bin every 100 steps
+             (MD-Bench's SORT_ATOMS option).
   eam-32     Verlet-list EAM, FP64: 32^3 Cu cells (131,072 atoms), a 3.615 A, synthetic funcfl
              tables (nr = nrho = 2000, rc 4.95 A), 1600 K, dt 1 fs, reneighbour every 10 steps.
              eam-64: the same at 64^3 cells (1,048,576 atoms).
@@ -99,9 +101,9 @@ def cpu_baseline_vl(nx, steps):
     return None
 
 
-def bench_vl(args, mod):
+def bench_vl(args, mod, resort=0):
     nx = 64
-    sim = mod.VLSim(nx=nx, ny=nx, nz=nx, ntimes=args.steps + args.warmup)
+    sim = mod.VLSim(nx=nx, ny=nx, nz=nx, ntimes=args.steps + args.warmup, resort_every=resort)
     try:
         natoms = sim.atom.Natoms
         sim.run(args.warmup)
@@ -137,8 +139,11 @@ def bench_vl(args, mod):
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "vl-lj-64: Verlet-list LJ FP64 full list, 64^3 fcc cells "
-                                   "(%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20" % natoms,
+            "config": {"workload": "vl-lj-64%s: Verlet-list LJ FP64 full list, 64^3 fcc cells "
+                                   "(%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20%s"
+                                   % ("-sort%d" % resort if resort else "", natoms,
+                                      ", atoms re-sorted by bin every %d steps" % resort
+                                      if resort else ""),
                        "atoms": natoms, "avg_neighbors": sum_nn / nlocal},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -603,6 +608,7 @@ WORKLOADS = {
     "cp-lj-64-4x8": lambda a, m: bench_cp(a, m, N=8),
     "cp-lj-64-half": lambda a, m: bench_cp(a, m, half=1),
     "vl-lj-64": bench_vl,
+    "vl-lj-64-sort100": lambda a, m: bench_vl(a, m, resort=100),
     "eam-32": bench_eam,
     "eam-64": lambda a, m: bench_eam(a, m, nx=64),
     "ka-128": bench_ka,

@@ -44,6 +44,10 @@ typedef struct {
     /* cluster-pair runs: prune the lists every this many steps between rebuilds (pruneNeighbor,
        src/clusterpair/main.c:329-332); 0 = the reference's default, 1000 */
     int prune_every;
+    /* Verlet-list runs: re-sort the local atoms by bin at setup and every this many steps
+       (SORT_ATOMS, src/verletlist/main.c:92-95, 372-381; the reference's default period is
+       400); 0 = no sorting, the reference's default build (config.mk:29) */
+    int resort_every;
 } mdb_config;
 
 /* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */

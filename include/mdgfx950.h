@@ -76,6 +76,13 @@ void setupNeighborGFX950(Parameter* param);
 /* replaces binatoms (src/verletlist/neighbor.c:431-464): counting sort, atoms in index order
    inside each bin */
 void binatomsGFX950(Atom* atom);
+/* replaces sortAtom (src/verletlist/neighbor.c:466-536), called under SORT_ATOMS at setup and
+   every resort_every steps (src/verletlist/main.c:92-95, 372-381): the local atoms re-stored in
+   bin order (bin-major, index order within a bin).  Sets Nghost = 0 as those callers do; ghosts
+   and the neighbour list must be rebuilt before the next force call.  Unlike the reference, which
+   moves positions and velocities only, each atom's force and type move with it, so that the kick
+   that follows uses the atom's own force. */
+void sortAtomGFX950(Atom* atom);
 /* replaces buildNeighborCUDA (src/verletlist/neighborCuda.cu:259-366) / buildNeighborCPU
    (src/verletlist/neighbor.c:260-352); grows neighbor->maxneighs on overflow */
 void buildNeighborGFX950(Atom* atom, Neighbor* neighbor);

@@ -47,6 +47,11 @@ struct State {
     // local positions there (its other lanes still read the current ones) and the two swap
     double* d_xn  = nullptr;
     size_t cap_xn = 0;
+    // atom sort: the bin-ordered copies of x, v and f (9 doubles per atom) and of the types
+    double* d_sort    = nullptr;
+    size_t cap_sort   = 0;
+    int* d_sort_t     = nullptr;
+    size_t cap_sort_t = 0;
     int ntypes = 0;
     bool uniform = true;  // every type pair has the same LJ parameters
     Tables tab{};
@@ -277,6 +282,25 @@ __global__ void k_bin_sort(int mbins, const int* __restrict__ binstart, int* __r
         }
         binatoms[q + 1] = v;
     }
+}
+
+// sortAtom (neighbor.c:466-536): new slot p takes the atom at position p of the bin-major list of
+// the local atoms (in index order within a bin), with its position, velocity, force and type.
+__global__ void k_sort_gather(int n, const int* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ v, const double* __restrict__ f, const int* __restrict__ type,
+    double* __restrict__ xs, double* __restrict__ vs, double* __restrict__ fs,
+    int* __restrict__ ts)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int o = order[p];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        xs[3 * p + d] = x[3 * o + d];
+        vs[3 * p + d] = v[3 * o + d];
+        fs[3 * p + d] = f[3 * o + d];
+    }
+    ts[p] = type[o];
 }
 
 // ---------------------------------------------------------------- neighbour build
@@ -812,6 +836,8 @@ void freeDeviceGFX950(Atom* atom, Neighbor* neighbor)
         neighbor->d_neighbor.numneigh  = nullptr;
     }
     dfree(S.d_xn);
+    dfree(S.d_sort);
+    dfree(S.d_sort_t);
     dfree(S.d_stencil);
     dfree(S.d_bin_of);
     dfree(S.d_bincount);
@@ -1001,6 +1027,41 @@ void binatomsGFX950(Atom* atom)
     k_bin_sort<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, S.d_binstart, S.d_binatoms);
     MDG_LAUNCH_CHECK("binatoms");
     S.binned = true;
+}
+
+void sortAtomGFX950(Atom* atom)
+{
+    // sortAtom (neighbor.c:466-536) with the caller's Nghost = 0 of main.c:92-95 and 372-381
+    // done here: the local atoms are binned alone and re-stored in bin order
+    if (failed() || !S.ready) return;
+    const int n  = atom->Nlocal;
+    atom->Nghost = 0;
+    if (n <= 0) return;
+    fill_i32(S.d_status, 0, 2);
+    binatomsGFX950(atom);
+    if (failed()) return;
+    read_status();
+    if (failed()) return;
+    if (S.h_status[0] & ST_BIN_RANGE) {
+        set_error(MDG_ERR_BIN, "sortAtomGFX950: an atom lies outside the bin grid");
+        return;
+    }
+    grow_n(&S.d_sort, &S.cap_sort, 9 * (size_t)n, 0);
+    grow_n(&S.d_sort_t, &S.cap_sort_t, (size_t)n, 0);
+    if (failed()) return;
+    DeviceAtom* d     = &atom->d_atom;
+    hipStream_t st    = stream();
+    double* xs        = S.d_sort;
+    double* vs        = S.d_sort + 3 * (size_t)n;
+    double* fs        = S.d_sort + 6 * (size_t)n;
+    k_sort_gather<<<blocks_for(n, BLOCK), BLOCK, 0, st>>>(n, S.d_binatoms, d->x, d->vx, d->fx,
+        d->type, xs, vs, fs, S.d_sort_t);
+    MDG_LAUNCH_CHECK("sort_gather");
+    MDG_HIP(hipMemcpyAsync(d->x, xs, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    MDG_HIP(hipMemcpyAsync(d->vx, vs, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    MDG_HIP(hipMemcpyAsync(d->fx, fs, 24 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    MDG_HIP(hipMemcpyAsync(d->type, S.d_sort_t, 4 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    S.binned = false;  // the bins name the old indices
 }
 
 void buildNeighborGFX950(Atom* atom, Neighbor* neighbor)

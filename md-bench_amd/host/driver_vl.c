@@ -22,6 +22,7 @@ struct mdb_vl {
     double* v0;
     int* t0;
     int device_ready;
+    int sort_on; /* SORT_ATOMS: re-sort by bin at setup and every param.resort_every steps */
     /* learned EAM surrogate (force_field FF_EAM only): judged by plain EAM every sur_every steps */
     int sur_on, sur_every, sur_fallback, sur_judges;
     double sur_tol, sur_last, sur_max;
@@ -82,6 +83,10 @@ int mdbVLCreate(const mdb_config* cfg, mdb_vl** out)
     p->ntimes        = cfg->ntimes;
     p->nstat         = cfg->nstat > 0 ? cfg->nstat : p->nstat;
     p->reneigh_every = cfg->reneigh_every > 0 ? cfg->reneigh_every : p->reneigh_every;
+    if (cfg->resort_every > 0) {
+        p->resort_every = cfg->resort_every;
+        s->sort_on      = 1;
+    }
     if (cfg->cutforce > 0) p->cutforce = cfg->cutforce;
     if (cfg->skin >= 0) p->skin = cfg->skin;
     if (cfg->ntypes > 0) p->ntypes = cfg->ntypes;
@@ -166,6 +171,7 @@ int mdbVLCreate(const mdb_config* cfg, mdb_vl** out)
     initDeviceGFX950(p, &s->atom, nb);
     s->device_ready = 1;
     setupNeighborGFX950(p);
+    if (s->sort_on) sortAtomGFX950(&s->atom); /* main.c:92-95, before setupPbc */
     if (p->force_field == FF_EAM)
         setEamTablesGFX950(s->eam.nr, s->eam.nrho, s->eam.rdr, s->eam.rdrho, s->eam.rhor_spline,
             s->eam.frho_spline, s->eam.z2r_spline, s->eam.nr_tot, s->eam.nrho_tot);
@@ -233,7 +239,12 @@ int mdbVLRun(mdb_vl* s, int nsteps, double* ksum)
     Neighbor* nb = &s->neighbor;
     int integrated = 0; /* this step's initialIntegrate ran inside the last force call */
     for (int n = 0; n < nsteps; n++) {
-        int reneigh = (n + 1) % p->reneigh_every == 0;
+        /* SORT_ATOMS (main.c:372-381): the sort comes before the step's initialIntegrate (so the
+           step before it does not fuse that integration into its force call) and the step then
+           rebuilds ghosts and lists, which the reference's default periods (400 and 20) give */
+        const int resort = s->sort_on && (n + 1) % p->resort_every == 0;
+        int reneigh      = resort || (n + 1) % p->reneigh_every == 0;
+        if (resort) sortAtomGFX950(a);
         if (!integrated) initialIntegrateGFX950(reneigh, p, a);
         if (reneigh) {
             updateAtomsPbcGFX950(a, p, true);
@@ -246,7 +257,8 @@ int mdbVLRun(mdb_vl* s, int nsteps, double* ksum)
         /* plain LJ/EAM steps carry the next step's initialIntegrate in the force call, unless
            this step's full-step velocities are read (after the last step, or every step when
            traced) or the surrogate runs (its judge reads this step's positions) */
-        integrated = n + 1 < nsteps && !ksum && !s->sur_on;
+        integrated = n + 1 < nsteps && !ksum && !s->sur_on &&
+                     !(s->sort_on && (n + 2) % p->resort_every == 0);
         if (integrated) {
             computeForceFinalInitialGFX950(p, a, nb, &s->stats);
             s->steps_done++;

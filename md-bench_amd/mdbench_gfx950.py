@@ -33,7 +33,7 @@ class Config(C.Structure):
         ("pair_epsilon", C.POINTER(C.c_double)), ("pair_sigma", C.POINTER(C.c_double)),
         ("pair_cutoff", C.POINTER(C.c_double)), ("type1_fraction", C.c_double),
         ("type_seed", C.c_int), ("input_file", C.c_char_p),
-        ("prune_every", C.c_int),
+        ("prune_every", C.c_int), ("resort_every", C.c_int),
     ]
 
 
@@ -135,6 +135,7 @@ def lib():
         getattr(L, fn).restype = C.c_double
     L.buildNeighborGFX950.argtypes = [vp, vp]
     L.binatomsGFX950.argtypes = [vp]
+    L.sortAtomGFX950.argtypes = [vp]
     L.setupPbcGFX950.argtypes = [vp, vp]
     L.updatePbcGFX950.argtypes = [vp, vp, C.c_bool]
     L.updateAtomsPbcGFX950.argtypes = [vp, vp, C.c_bool]
@@ -273,6 +274,13 @@ class VLSim:
         x, v, t = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n, np.int32)
         lib().mdbVLInitialState(self.h, x, v, t)
         return x, v, t
+
+    def types(self):
+        """The local atoms' types in the device's current order (copyDataFromGFX950Device)."""
+        L = lib()
+        L.copyDataFromGFX950Device(self.param_p, self.atom_p)
+        n = self.atom.Nlocal
+        return np.ctypeslib.as_array(self.atom.type, (n,)).copy()
 
     def state(self, lists=True):
         """Download x (local+ghost), f, v, ghost map and the neighbour list (CSR)."""

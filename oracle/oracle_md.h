@@ -17,6 +17,14 @@
  *                                                            finite-difference -dE/dx
  *   orc_integrate_*                                          closed forms of tests/test_integrate.c
  *   orc_vl_run                                               golden trajectories; T/P at step 200
+ *   orc_vl_sort_atoms / orc_vl_run_resort                    properties that fix the order: a
+ *                                                            permutation, bins non-decreasing,
+ *                                                            index order inside a bin, v/f/type
+ *                                                            carried with x; bins pinned above;
+ *                                                            a sorted run conserves what an
+ *                                                            unsorted one does (no golden data:
+ *                                                            SORT_ATOMS is off in the reference's
+ *                                                            default build)
  *   orc_cp_*                                                 see oracle_cp.c header
  */
 #ifndef ORACLE_MD_H
@@ -47,6 +55,9 @@ int orc_vl_setup_grid_box(orc_vl_grid* g, const double* box, double cutneigh);
 int orc_vl_coord2bin(const orc_vl_grid* g, double x, double y, double z);
 /* Bins all nall atoms; bins is CSR (binstart[mbins+1], binatoms[nall]) in atom-index order. */
 void orc_vl_binatoms(const orc_vl_grid* g, int nall, const double* x, int* binstart, int* binatoms);
+/* sortAtom (neighbor.c:466-536) on the local atoms; f and type (either may be NULL) move too */
+void orc_vl_sort_atoms(const orc_vl_grid* g, int nlocal, double* x, double* v, double* f,
+    int* type);
 /* Neighbour list as CSR: numneigh[nlocal], neighbors[cap]. Returns total count or -1 if cap
    exceeded. */
 long long orc_vl_build_neighbor(const orc_vl_grid* g, int nlocal, int nall, const double* x,
@@ -106,6 +117,12 @@ double orc_kinetic_sum(int n, const double* v, double mass);
 /* Whole Verlet-list trajectory: setup from given x,v,type (already adjusted), then the loop of
    src/verletlist/main.c:371-419.  ksum[ntimes+1] receives sum m v^2 after every step.
    force_field 0 = LJ, 1 = EAM (eam must be set).  Returns 0 on success. */
+/* orc_vl_run with SORT_ATOMS every resort_every steps (0 = never); type_out may be NULL */
+int orc_vl_run_resort(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
+    const int* type, int ntypes, const double* epsilon, const double* sigma6,
+    const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
+    int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
+    int reneigh_every, double mass, double* f, double* ksum, int resort_every, int* type_out);
 int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
     const int* type, int ntypes, const double* epsilon, const double* sigma6,
     const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,

@@ -154,6 +154,37 @@ void orc_vl_binatoms(const orc_vl_grid* g, int nall, const double* x, int* binst
     free(bin);
 }
 
+/* sortAtom: neighbor.c:466-536, with the caller's Nghost = 0 (main.c:92-95, 372-381): the local
+   atoms binned alone and re-stored bin by bin, in binatoms order within a bin.  The reference
+   moves x and v only; here f and type move too, so that the following kick uses each atom's own
+   force (the product does the same). */
+void orc_vl_sort_atoms(const orc_vl_grid* g, int nlocal, double* x, double* v, double* f,
+    int* type)
+{
+    int* binstart = malloc(sizeof(int) * (g->mbins + 1));
+    int* order    = malloc(sizeof(int) * (nlocal + 1));
+    double* tmp   = malloc(sizeof(double) * 3 * (nlocal + 1));
+    int* ttmp     = malloc(sizeof(int) * (nlocal + 1));
+    orc_vl_binatoms(g, nlocal, x, binstart, order);
+    double* arr[3] = { x, v, f };
+    for (int a = 0; a < 3; a++) {
+        if (!arr[a]) continue;
+        for (int p = 0; p < nlocal; p++)
+            for (int d = 0; d < 3; d++)
+                tmp[3 * p + d] = arr[a][3 * order[p] + d];
+        memcpy(arr[a], tmp, sizeof(double) * 3 * nlocal);
+    }
+    if (type) {
+        for (int p = 0; p < nlocal; p++)
+            ttmp[p] = type[order[p]];
+        memcpy(type, ttmp, sizeof(int) * nlocal);
+    }
+    free(binstart);
+    free(order);
+    free(tmp);
+    free(ttmp);
+}
+
 /* buildNeighborCPU: neighbor.c:260-352 (half_stencil = 0, no shell method). */
 long long orc_vl_build_neighbor(const orc_vl_grid* g, int nlocal, int nall, const double* x,
     const int* type, int ntypes, const double* cutneighsq, int half, int* numneigh,
@@ -361,7 +392,7 @@ static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
     const int* type, int ntypes, const double* epsilon, const double* sigma6,
     const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
     int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
-    int reneigh_every, double mass, double* f, double* ksum);
+    int reneigh_every, double mass, double* f, double* ksum, int resort_every, int* type_out);
 
 int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
     const int* type, int ntypes, const double* epsilon, const double* sigma6,
@@ -372,7 +403,8 @@ int orc_vl_run(int nx, int ny, int nz, double lattice, int nlocal, double* x, do
     orc_vl_grid g;
     if (orc_vl_setup_grid(&g, nx, ny, nz, lattice, cutneigh)) return -1;
     return run_on_grid(&g, nlocal, x, v, type, ntypes, epsilon, sigma6, cutforcesq, cutneighsq,
-        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum);
+        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum, 0,
+        NULL);
 }
 
 int orc_vl_run_box(const double* box, int nlocal, double* x, double* v, const int* type,
@@ -384,14 +416,31 @@ int orc_vl_run_box(const double* box, int nlocal, double* x, double* v, const in
     orc_vl_grid g;
     if (orc_vl_setup_grid_box(&g, box, cutneigh)) return -1;
     return run_on_grid(&g, nlocal, x, v, type, ntypes, epsilon, sigma6, cutforcesq, cutneighsq,
-        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum);
+        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum, 0,
+        NULL);
+}
+
+/* The run loop with SORT_ATOMS (main.c:92-95, 372-381): sorted at setup and every resort_every
+   steps before the step's initialIntegrate, and that step rebuilds.  x, v (and f, type_out when
+   given) come back in the final order. */
+int orc_vl_run_resort(int nx, int ny, int nz, double lattice, int nlocal, double* x, double* v,
+    const int* type, int ntypes, const double* epsilon, const double* sigma6,
+    const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
+    int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
+    int reneigh_every, double mass, double* f, double* ksum, int resort_every, int* type_out)
+{
+    orc_vl_grid g;
+    if (orc_vl_setup_grid(&g, nx, ny, nz, lattice, cutneigh)) return -1;
+    return run_on_grid(&g, nlocal, x, v, type, ntypes, epsilon, sigma6, cutforcesq, cutneighsq,
+        cutneigh, half, force_field, eam, dt, dtforce, ntimes, reneigh_every, mass, f, ksum,
+        resort_every, type_out);
 }
 
 static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
     const int* type, int ntypes, const double* epsilon, const double* sigma6,
     const double* cutforcesq, const double* cutneighsq, double cutneigh, int half,
     int force_field, const orc_eam* eam, double dt, double dtforce, int ntimes,
-    int reneigh_every, double mass, double* f, double* ksum)
+    int reneigh_every, double mass, double* f, double* ksum, int resort_every, int* type_out)
 {
     const orc_vl_grid g = *gp;
     int gcap         = 27 * nlocal + 1024;
@@ -436,11 +485,14 @@ static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
                 NULL);                                                                           \
     } while (0)
 
+    if (resort_every > 0) orc_vl_sort_atoms(&g, nlocal, xa, v, NULL, ta);
     ORC_RENEIGH();
     ORC_FORCE();
     if (ksum) ksum[0] = orc_kinetic_sum(nlocal, v, mass);
     for (int n = 0; n < ntimes; n++) {
-        int reneigh = (n + 1) % reneigh_every == 0;
+        const int resort = resort_every > 0 && (n + 1) % resort_every == 0;
+        int reneigh      = resort || (n + 1) % reneigh_every == 0;
+        if (resort) orc_vl_sort_atoms(&g, nlocal, xa, v, fa, ta);
         orc_initial_integrate(nlocal, xa, v, fa, dt, dtforce);
         if (reneigh) {
             orc_update_atoms_pbc(nlocal, xa, g.xprd, g.yprd, g.zprd);
@@ -454,6 +506,7 @@ static int run_on_grid(const orc_vl_grid* gp, int nlocal, double* x, double* v,
     }
     memcpy(x, xa, sizeof(double) * 3 * nlocal);
     if (f) memcpy(f, fa, sizeof(double) * 3 * nlocal);
+    if (type_out) memcpy(type_out, ta, sizeof(int) * nlocal);
 #undef ORC_RENEIGH
 #undef ORC_FORCE
 done:

@@ -79,6 +79,8 @@ def lib():
     L.orc_vl_run_box.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp,
                                  C.c_double, C.c_int, C.c_int, C.POINTER(Eam), C.c_double,
                                  C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp]
+    L.orc_vl_sort_atoms.argtypes = [C.POINTER(VLGrid), C.c_int, _dp, _dp, _dp, _ip]
+    L.orc_vl_run_resort.argtypes = L.orc_vl_run.argtypes + [C.c_int, _ip]
     _lib = L
     return L
 
@@ -208,6 +210,36 @@ def vl_run(nx, ny, nz, lattice, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutn
                           dt, dtforce, ntimes, reneigh, mass, f, ks)
     assert rc == 0, rc
     return xx, vv, f, ks
+
+
+def vl_sort_atoms(g, x, v, f, type_):
+    """sortAtom on the local atoms: the four arrays come back in bin order."""
+    x, v, f = (np.ascontiguousarray(a, np.float64).copy() for a in (x, v, f))
+    t = np.ascontiguousarray(type_, np.int32).copy()
+    lib().orc_vl_sort_atoms(C.byref(g), x.size // 3, x, v, f, t)
+    return x, v, f, t
+
+
+def vl_run_resort(nx, ny, nz, lattice, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutneigh,
+                  half, ntimes, reneigh, dt, dtforce, mass, resort, eam=None):
+    """vl_run with SORT_ATOMS every `resort` steps; also returns the final types."""
+    nlocal = x.size // 3
+    xx = np.ascontiguousarray(x, np.float64).copy()
+    vv = np.ascontiguousarray(v, np.float64).copy()
+    f = np.zeros(3 * nlocal)
+    ks = np.zeros(ntimes + 1)
+    tt = np.zeros(nlocal, np.int32)
+    rc = lib().orc_vl_run_resort(nx, ny, nz, lattice, nlocal, xx, vv,
+                                 np.ascontiguousarray(type_, np.int32), ntypes,
+                                 np.ascontiguousarray(eps, np.float64),
+                                 np.ascontiguousarray(sig6, np.float64),
+                                 np.ascontiguousarray(cutf, np.float64),
+                                 np.ascontiguousarray(cutn, np.float64), cutneigh, int(half),
+                                 1 if eam is not None else 0,
+                                 C.byref(eam.eam) if eam is not None else None,
+                                 dt, dtforce, ntimes, reneigh, mass, f, ks, resort, tt)
+    assert rc == 0, rc
+    return xx, vv, f, ks, tt
 
 
 def vl_run_box(box, x, v, type_, ntypes, eps, sig6, cutf, cutn, cutneigh, half, ntimes,

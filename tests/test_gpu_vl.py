@@ -33,6 +33,17 @@ def _cfg_for(d, strict):
     return kw
 
 
+def _eam_file(arg):
+    """The funcfl file behind a golden EAM case: the synthetic table regenerated, or the fixture."""
+    import tempfile
+    from golden.eam_tables import write_funcfl_synthetic
+    if "Cu_u3" in arg:
+        return os.path.join(REPO, "tests", "golden", "Cu_u3.eam")
+    path = os.path.join(tempfile.gettempdir(), "mdb_cu_synth_sort_%d.eam" % os.getpid())
+    write_funcfl_synthetic(path)
+    return path
+
+
 def _relerr(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
@@ -161,14 +172,16 @@ def test_default_testcase_regression(strict):
 
 
 @pytest.mark.parametrize("ff", [0, 1], ids=["lj", "eam"])
-def test_fused_integration_equals_unfused_run(ff):
+@pytest.mark.parametrize("resort", [0, 9], ids=["unsorted", "resort9"])
+def test_fused_integration_equals_unfused_run(ff, resort):
     """The run loop integrates the next step inside the force call (computeForceFinalInitial-
     GFX950, two position buffers) except where a step's full-step velocities are read; a traced
     run, which reads them every step and so fuses nothing, gives the same state bit for bit,
-    across rebuilds every 7 steps.  (Half lists run the separate calls; their atomic force sums
-    have no fixed order, so two half-list runs agree to rounding only.)"""
+    across rebuilds every 7 steps, and with the atoms re-sorted every 9 (the step before a sort
+    does not fuse).  (Half lists run the separate calls; their atomic force sums have no fixed
+    order, so two half-list runs agree to rounding only.)"""
     P = product()
-    kw = dict(nx=6, ny=6, nz=6, ntimes=0, reneigh_every=7)
+    kw = dict(nx=6, ny=6, nz=6, ntimes=0, reneigh_every=7, resort_every=resort)
     if ff:
         kw.update(force_field=1, lattice_a=3.615, temp=1600.0)
     out = []
@@ -280,3 +293,39 @@ def test_synthetic_pattern_benchmark_forces_match_oracle(pattern):
         assert res["f0"] == f[0]
     assert res["flast"] == f[3 * (n - 1)] and np.isfinite(res["flast"])
     assert res["atom_updates_per_s"] > 0
+
+
+@pytest.mark.parametrize("case", ["vl_lj_types", "vl_eam_synth_4"])
+def test_sorted_run_matches_oracle(case):
+    """SORT_ATOMS (sortAtomGFX950: setup and every 7 steps, rebuilds every 5, so some sorts fall
+    between rebuilds and force one): the strict kernels give the oracle's sorted run bit for bit,
+    positions, velocities, forces and the types that moved with the atoms."""
+    print("case", case, flush=True)
+    d = load_golden(case)
+    args = [str(a) for a in d["args"]]
+    nx, ny, nz = int(args[0]), int(args[1]), int(args[2])
+    kw = _cfg_for(d, 1)
+    kw.update(half_neigh=0, ntimes=0, reneigh_every=5, resort_every=7)
+    tab = None
+    if args[6] == "eam":
+        tab = ob.EamTables(_eam_file(args[7]))
+    P = product()
+    sim = P.VLSim(**kw)
+    try:
+        x0, v0, t0 = sim.initial_state()
+        sim.run(30)
+        st = sim.state(lists=False)
+        types = sim.types()
+        nl = st["nlocal"]
+    finally:
+        sim.close()
+    x, v, f, ks, t = ob.vl_run_resort(nx, ny, nz, meta(d, "lattice"), x0, v0, t0,
+                                      meta(d, "ntypes"), d["epsilon"], d["sigma6"],
+                                      d["cutforcesq"], d["cutneighsq"], meta(d, "cutneigh"), 0,
+                                      30, 5, meta(d, "dt"), meta(d, "dtforce"), meta(d, "mass"), 7,
+                                      eam=tab)
+    assert not np.array_equal(t, t0) or case != "vl_lj_types"
+    np.testing.assert_array_equal(types, t)
+    np.testing.assert_array_equal(st["x"][:3 * nl], x)
+    np.testing.assert_array_equal(st["v"], v)
+    np.testing.assert_array_equal(st["f"], f)

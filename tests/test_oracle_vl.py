@@ -4,6 +4,7 @@ Golden vectors come from MD-Bench's own Verlet-list sources built with strict IE
 (tests/golden/make_golden.py).  Bins, ghost maps and neighbour lists are integer work and must be
 identical; forces and trajectories of the strict restatement must be bit-identical too.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -155,3 +156,61 @@ def test_eam_atom_energy_sums_to_total_and_is_the_force_potential(case):
                 es.append(ob.force_eam_atom(tab, nlocal, xs, bmap, cutf, nn, nb)[1].sum())
             fd = -(es[0] - es[1]) / (2 * h)
             assert abs(fd - f[3 * k + c]) <= 1e-6 * max(1.0, abs(f[3 * k + c])), (k, c, fd)
+
+
+def _bins(g, x):
+    L = ob.lib()
+    return np.array([L.orc_vl_coord2bin(C.byref(g), *x[3 * i:3 * i + 3])
+                     for i in range(x.size // 3)])
+
+
+def test_sort_atoms_is_the_stable_bin_order():
+    """sortAtom (neighbor.c:466-536): the new order is the one permutation that makes the bins
+    non-decreasing and keeps index order inside a bin, and v, f and type travel with x."""
+    d = load_golden("vl_lj_4")
+    g = _grid(d)
+    rng = np.random.default_rng(11)
+    n = 400
+    x = rng.uniform(0.0, g.xprd, 3 * n)
+    v, f = rng.normal(size=3 * n), rng.normal(size=3 * n)
+    t = np.arange(n, dtype=np.int32)  # types as atom ids
+    xs, vs, fs, ts = ob.vl_sort_atoms(g, x, v, f, t)
+    perm = ts.astype(np.int64)
+    assert sorted(perm.tolist()) == list(range(n))
+    for a, b in ((xs, x), (vs, v), (fs, f)):
+        np.testing.assert_array_equal(a.reshape(-1, 3), b.reshape(-1, 3)[perm])
+    b = _bins(g, x)[perm]
+    assert np.all(np.diff(b) >= 0)
+    same = np.diff(b) == 0
+    assert np.all(np.diff(perm)[same] > 0)
+    # the binning behind it is the golden-pinned one: bin-major order of orc_vl_binatoms
+    start, atoms = ob.binatoms(g, x)
+    np.testing.assert_array_equal(perm, atoms)
+    # sorting again changes nothing
+    again = ob.vl_sort_atoms(g, xs, vs, fs, ts)
+    for a, b2 in zip(again, (xs, vs, fs, ts)):
+        np.testing.assert_array_equal(a, b2)
+
+
+def test_resorted_run_follows_the_unsorted_one():
+    """A run that re-sorts (at setup and every 7 steps, rebuilding every 5) is the unsorted run
+    with its atoms relabelled: with one parameter set for every type pair and the types as atom
+    ids, each atom ends where the unsorted run puts it (to the rounding that the changed order of
+    the force sums makes) and the kinetic-energy trace agrees."""
+    d = load_golden("vl_lj_4")
+    nx, ny, nz, half = _dims(d)
+    n = d["create_x"].size // 3
+    nt = n
+    one = np.ones(nt * nt)
+    ids = np.arange(n, dtype=np.int32)
+    common = (nx, ny, nz, meta(d, "lattice"), d["create_x"], d["adjust_v"], ids, nt,
+              one * d["epsilon"][0], one * d["sigma6"][0], one * d["cutforcesq"][0],
+              one * d["cutneighsq"][0], meta(d, "cutneigh"), 0, 30, 5, meta(d, "dt"),
+              meta(d, "dtforce"), meta(d, "mass"))
+    x0, v0, f0, k0 = ob.vl_run(*common)
+    x1, v1, f1, k1, t1 = ob.vl_run_resort(*common, 7)
+    assert sorted(t1.tolist()) == list(range(n)) and not np.array_equal(t1, ids)
+    np.testing.assert_allclose(k1, k0, rtol=1e-12)
+    np.testing.assert_allclose(x1.reshape(-1, 3), x0.reshape(-1, 3)[t1], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(v1.reshape(-1, 3), v0.reshape(-1, 3)[t1], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(f1.reshape(-1, 3), f0.reshape(-1, 3)[t1], rtol=0, atol=1e-8)
