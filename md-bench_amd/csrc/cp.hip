@@ -1381,8 +1381,9 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                 if (below && !in) in = rsqd < cut2;
             }
             if (SELF || EV) in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
-            // v_rcp_f32 alone: with a Newton step the 1M-atom force error is 5.7e-7, without it
-            // 6.0e-7 (measured); the step cost two of the ~25 instructions of a pair
+            // v_rcp_f32 alone: with a Newton step the 1M-atom force error was 5.7e-7, without it
+            // 6.0e-7 (measured with the rounded FP64 displacement); the step cost two of the ~25
+            // instructions of a pair
             const float r2i = __builtin_amdgcn_rcpf(rsq);
             const float r6i = r2i * r2i * r2i;
             float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
