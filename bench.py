@@ -17,7 +17,8 @@ Workloads (seeded fcc lattices, synthetic velocities):
              (MD-Bench's SORT_ATOMS option).
   eam-32     Verlet-list EAM, FP64: 32^3 Cu cells (131,072 atoms), a 3.615 A, synthetic funcfl
              tables (nr = nrho = 2000, rc 4.95 A), 1600 K, dt 1 fs, reneighbour every 10 steps.
-             eam-64: the same at 64^3 cells (1,048,576 atoms).
+             eam-64: the same at 64^3 cells (1,048,576 atoms); eam-64-sort100 with the atoms
+             re-sorted by bin every 100 steps.
   ka-128     Kob-Andersen 80/20 binary LJ mixture, FP32-mixed cluster pair 4x4: 128^3 cells
              (8,388,608 atoms), rho 1.2, T 0.8, per-pair cutoff 2.5 sigma_ab.
   eam-surrogate-64  the learned EAM surrogate: trained on 2000 plain-EAM snapshots of 4000-atom
@@ -265,10 +266,10 @@ def cpu_baseline_eam(nx, steps):
                       % (nx, 4 * nx ** 3, steps)}
 
 
-def bench_eam(args, mod, nx=32):
+def bench_eam(args, mod, nx=32, resort=0):
     import ctypes as C
     sim = mod.VLSim(nx=nx, ny=nx, nz=nx, force_field=1, lattice_a=3.615, temp=1600.0,
-                    reneigh_every=10, ntimes=args.steps + args.warmup)
+                    reneigh_every=10, ntimes=args.steps + args.warmup, resort_every=resort)
     try:
         natoms = sim.atom.Natoms
         sim.run(args.warmup)
@@ -304,9 +305,12 @@ def bench_eam(args, mod, nx=32):
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "eam-%d: Verlet-list EAM FP64, %d^3 Cu fcc cells (%d atoms), "
+            "config": {"workload": "eam-%d%s: Verlet-list EAM FP64, %d^3 Cu fcc cells (%d atoms), "
                                    "a 3.615 A, synthetic funcfl nr=nrho=2000 rc 4.95 A, 1600 K, "
-                                   "dt 1 fs, reneigh 10" % (nx, nx, natoms),
+                                   "dt 1 fs, reneigh 10%s"
+                                   % (nx, "-sort%d" % resort if resort else "", nx, natoms,
+                                      ", atoms re-sorted by bin every %d steps" % resort
+                                      if resort else ""),
                        "atoms": natoms, "avg_neighbors": sum_nn / nlocal},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -611,6 +615,7 @@ WORKLOADS = {
     "vl-lj-64-sort100": lambda a, m: bench_vl(a, m, resort=100),
     "eam-32": bench_eam,
     "eam-64": lambda a, m: bench_eam(a, m, nx=64),
+    "eam-64-sort100": lambda a, m: bench_eam(a, m, nx=64, resort=100),
     "ka-128": bench_ka,
     "eam-surrogate-64": bench_surrogate,
     "stub-seq": lambda a, m: bench_stub(a, m, "seq"),
