@@ -53,6 +53,7 @@ struct State {
     int* d_sort_t     = nullptr;
     size_t cap_sort_t = 0;
     int ntypes = 0;
+    int ncu    = 0;  // compute units, rounded up to a multiple of 8 (persistent EAM grids)
     bool uniform = true;  // every type pair has the same LJ parameters
     Tables tab{};
     // bins
@@ -548,6 +549,39 @@ __global__ void __launch_bounds__(BLOCK) k_force_lj(int nlocal, const double* __
 }
 
 // ---------------------------------------------------------------- EAM (force_eam.c:19-228)
+// Spline knot of a pair distance (force_eam.c:72-76, 154-158): row m and offset p in [0, 1].
+__device__ inline int eam_knot(double r, double rdr, int nr, double* p)
+{
+    double pp = r * rdr + 1.0;
+    int m     = (int)pp;
+    m         = m < nr - 1 ? m : nr - 1;
+    pp -= m;
+    *p = pp < 1.0 ? pp : 1.0;
+    return m;
+}
+
+// rho_j(r) from the value coefficients (c3..c6) of its row (force_eam.c:85-88)
+__device__ inline double eam_rho(const double* c, double p)
+{
+    return ((c[0] * p + c[1]) * p + c[2]) * p + c[3];
+}
+
+// The pair force over r (force_eam.c:188-206) from the derivative coefficients c0..c2 of the rho
+// row (cr) and the full z2r row (cz); phi for the energy
+__device__ inline double eam_fpair(const double* cr, const double* cz, double p, double r,
+    double fpi, double fpj, double* phi_out)
+{
+    double rhoip = (cr[0] * p + cr[1]) * p + cr[2];
+    double z2p   = (cz[0] * p + cz[1]) * p + cz[2];
+    double z2    = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
+    double recip = 1.0 / r;
+    double phi   = z2 * recip;
+    double phip  = z2p * recip - phi * recip;
+    double psip  = fpi * rhoip + fpj * rhoip + phip;
+    *phi_out     = phi;
+    return -psip * recip;
+}
+
 template <int LPA, bool STRICT, bool EV>
 __global__ void __launch_bounds__(BLOCK) k_eam_density(int nlocal, const double* __restrict__ x,
     const int* __restrict__ neighbors, const int* __restrict__ numneigh, double cutforcesq,
@@ -570,13 +604,9 @@ __global__ void __launch_bounds__(BLOCK) k_eam_density(int nlocal, const double*
             double dz   = zt - x[3 * j + 2];
             double rsq  = STRICT ? dx * dx + dy * dy + dz * dz : fma(dx, dx, fma(dy, dy, dz * dz));
             if (rsq < cutforcesq) {
-                double p = sqrt(rsq) * rdr + 1.0;
-                int m    = (int)p;
-                m        = m < nr - 1 ? m : nr - 1;
-                p -= m;
-                p                 = p < 1.0 ? p : 1.0;
-                const double* c   = rs + m * 7;
-                rhoi += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+                double p;
+                const int m = eam_knot(sqrt(rsq), rdr, nr, &p);
+                rhoi += eam_rho(rs + m * 7 + 3, p);
             }
         }
     }
@@ -641,21 +671,9 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
             double rsq  = STRICT ? dx * dx + dy * dy + dz * dz : fma(dx, dx, fma(dy, dy, dz * dz));
             if (rsq < cutforcesq) {
                 double r = sqrt(rsq);
-                double p = r * rdr + 1.0;
-                int m    = (int)p;
-                m        = m < nr - 1 ? m : nr - 1;
-                p -= m;
-                p                = p < 1.0 ? p : 1.0;
-                const double* cr = rs + m * 7;
-                const double* cz = zs + m * 7;
-                double rhoip     = (cr[0] * p + cr[1]) * p + cr[2];
-                double z2p       = (cz[0] * p + cz[1]) * p + cz[2];
-                double z2        = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
-                double recip     = 1.0 / r;
-                double phi       = z2 * recip;
-                double phip      = z2p * recip - phi * recip;
-                double psip      = fpi * rhoip + fp[j] * rhoip + phip;
-                double fpair     = -psip * recip;
+                double p, phi;
+                const int m        = eam_knot(r, rdr, nr, &p);
+                const double fpair = eam_fpair(rs + m * 7, zs + m * 7, p, r, fpi, fp[j], &phi);
                 if (STRICT) {
                     fix += dx * fpair;
                     fiy += dy * fpair;
@@ -692,6 +710,136 @@ __global__ void __launch_bounds__(BLOCK) k_eam_force(int nlocal, const double* _
         if (threadIdx.x == 0) {
             atomicAdd(&ev[0], be);
             atomicAdd(&ev[1], bw);
+        }
+    }
+}
+
+// The same two passes with the spline rows staged in LDS (no energy): the per-pair table rows
+// (32 B in the density pass, 80 B in the force pass) then come from LDS, and the texture path
+// serves only the neighbour gathers.  The tables take 64 KB and 160,000 B for nr <= TAB_NR, so
+// the blocks are persistent: TAB_BLOCK threads, one block per 64 KB / 160 KB of a CU's LDS,
+// each over a contiguous range of atoms (ranges of one XCD adjacent, as in the cluster-pair
+// kernels).  Operations and their order are those of k_eam_density / k_eam_force.
+constexpr int TAB_BLOCK = 1024;
+constexpr int TAB_NR    = 2000;
+
+template <int LPA, bool STRICT>
+__global__ void __launch_bounds__(TAB_BLOCK) k_eam_density_tab(int nlocal, int per_block,
+    const double* __restrict__ x, const int* __restrict__ neighbors,
+    const int* __restrict__ numneigh, double cutforcesq, const double* __restrict__ rs,
+    const double* __restrict__ fs, int nr, int nrho, double rdr, double rdrho,
+    double* __restrict__ fp)
+{
+    __shared__ double s_rs[TAB_NR * 4];  // value coefficients c3..c6 of each rho row
+    for (int k = threadIdx.x; k < nr * 4; k += TAB_BLOCK)
+        s_rs[k] = rs[(k >> 2) * 7 + 3 + (k & 3)];
+    __syncthreads();
+    const int per   = gridDim.x >> 3;
+    const int lb    = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int begin = lb * per_block;
+    const int end   = min(nlocal, begin + per_block);
+    const int lane  = threadIdx.x % LPA;
+    for (int base = begin; base < end; base += TAB_BLOCK / LPA) {
+        const int i    = base + threadIdx.x / LPA;
+        const bool act = i < end;
+        double rhoi    = 0;
+        if (act) {
+            const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
+            const int nn    = numneigh[i];
+            for (int k = lane; k < nn; k += LPA) {
+                const int j = neighbors[(size_t)k * nlocal + i];
+                double dx   = xt - x[3 * j];
+                double dy   = yt - x[3 * j + 1];
+                double dz   = zt - x[3 * j + 2];
+                double rsq  = STRICT ? dx * dx + dy * dy + dz * dz : fma(dx, dx, fma(dy, dy, dz * dz));
+                if (rsq < cutforcesq) {
+                    double p;
+                    const int m = eam_knot(sqrt(rsq), rdr, nr, &p);
+                    rhoi += eam_rho(s_rs + m * 4, p);
+                }
+            }
+        }
+        if (LPA > 1) rhoi = group_sum<LPA>(rhoi);
+        if (act && lane == 0) {
+            double p = 1.0 * rhoi * rdrho + 1.0;
+            int m    = (int)p;
+            m        = m < nrho - 1 ? m : nrho - 1;
+            m        = m > 1 ? m : 1;
+            p -= m;
+            p               = p < 1.0 ? p : 1.0;
+            const double* c = fs + m * 7;
+            fp[i]           = (c[0] * p + c[1]) * p + c[2];
+        }
+    }
+}
+
+template <int LPA, bool STRICT>
+__global__ void __launch_bounds__(TAB_BLOCK) k_eam_force_tab(int nlocal, int per_block,
+    const double* __restrict__ x, const int* __restrict__ neighbors,
+    const int* __restrict__ numneigh, double cutforcesq, const double* __restrict__ rs,
+    const double* __restrict__ zs, int nr, double rdr, const double* __restrict__ fp,
+    double* __restrict__ f, Fuse fu)
+{
+    __shared__ double s_t[TAB_NR * 10];  // per row: rho c0..c2, then z2r c0..c6
+    for (int k = threadIdx.x; k < nr * 10; k += TAB_BLOCK) {
+        const int row = k / 10, c = k - 10 * row;
+        s_t[k]        = c < 3 ? rs[row * 7 + c] : zs[row * 7 + c - 3];
+    }
+    __syncthreads();
+    const int per   = gridDim.x >> 3;
+    const int lb    = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int begin = lb * per_block;
+    const int end   = min(nlocal, begin + per_block);
+    const int lane  = threadIdx.x % LPA;
+    for (int base = begin; base < end; base += TAB_BLOCK / LPA) {
+        const int i    = base + threadIdx.x / LPA;
+        const bool act = i < end;
+        double fix = 0, fiy = 0, fiz = 0;
+        double v0[3] = {}, x0[3] = {};
+        if (fu.v && act && lane == 0)
+            for (int d = 0; d < 3; d++)
+                v0[d] = fu.v[3 * i + d];
+        if (act) {
+            const double xt = x[3 * i], yt = x[3 * i + 1], zt = x[3 * i + 2];
+            x0[0]            = xt;
+            x0[1]            = yt;
+            x0[2]            = zt;
+            const double fpi = fp[i];
+            const int nn     = numneigh[i];
+            for (int k = lane; k < nn; k += LPA) {
+                const int j = neighbors[(size_t)k * nlocal + i];
+                double dx   = xt - x[3 * j];
+                double dy   = yt - x[3 * j + 1];
+                double dz   = zt - x[3 * j + 2];
+                double rsq  = STRICT ? dx * dx + dy * dy + dz * dz : fma(dx, dx, fma(dy, dy, dz * dz));
+                if (rsq < cutforcesq) {
+                    double r = sqrt(rsq);
+                    double p, phi;
+                    const int m        = eam_knot(r, rdr, nr, &p);
+                    const double* row  = s_t + m * 10;
+                    const double fpair = eam_fpair(row, row + 3, p, r, fpi, fp[j], &phi);
+                    if (STRICT) {
+                        fix += dx * fpair;
+                        fiy += dy * fpair;
+                        fiz += dz * fpair;
+                    } else {
+                        fix = fma(dx, fpair, fix);
+                        fiy = fma(dy, fpair, fiy);
+                        fiz = fma(dz, fpair, fiz);
+                    }
+                }
+            }
+        }
+        if (LPA > 1) {
+            fix = group_sum<LPA>(fix);
+            fiy = group_sum<LPA>(fiy);
+            fiz = group_sum<LPA>(fiz);
+        }
+        if (act && lane == 0) {
+            f[3 * i]     = fix;
+            f[3 * i + 1] = fiy;
+            f[3 * i + 2] = fiz;
+            if (fu.v) fused_integrate(fu, i, v0, x0, fix, fiy, fiz);
         }
     }
 }
@@ -756,6 +904,13 @@ void initDeviceGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
 {
     if (failed() || !stream()) return;
     if (!ensure_status()) return;
+    {
+        int dev = 0, cu = 0;
+        MDG_HIP(hipGetDevice(&dev));
+        MDG_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (failed()) return;
+        S.ncu = (cu + 7) / 8 * 8;
+    }
     DeviceAtom* d   = &atom->d_atom;
     memset(d, 0, sizeof(*d));
     S.cap_all = S.cap_v = 0;
@@ -1266,13 +1421,30 @@ static double force_eam(Parameter* param, Atom* atom, Neighbor* neighbor, Fuse f
             neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
             S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, S.d_ev, eatom, fu);                   \
     } while (0)
+    // without the energy and for tables of up to TAB_NR rows: the LDS-table passes
+#define MDG_EAM_TAB(L, ST)                                                                       \
+    do {                                                                                         \
+        const unsigned gd = 2u * (unsigned)S.ncu, gf = (unsigned)S.ncu;                          \
+        const int pd = (int)((nlocal + gd - 1) / gd), pf = (int)((nlocal + gf - 1) / gf);        \
+        k_eam_density_tab<L, ST><<<gd, TAB_BLOCK, 0, s>>>(nlocal, pd, atom->d_atom.x,            \
+            neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
+            S.d_frho, S.nr, S.nrho, S.rdr, S.rdrho, S.d_fp);                                     \
+        launch_ghost_copy(S.d_fp, atom);                                                         \
+        k_eam_force_tab<L, ST><<<gf, TAB_BLOCK, 0, s>>>(nlocal, pf, atom->d_atom.x,              \
+            neighbor->d_neighbor.neighbors, neighbor->d_neighbor.numneigh, cutf, S.d_rhor,       \
+            S.d_z2r, S.nr, S.rdr, S.d_fp, atom->d_atom.fx, fu);                                  \
+    } while (0)
+    const bool tab = !ev && S.nr <= TAB_NR && S.ncu > 0 && nlocal > 0;
     if (strict) {
         if (ev) MDG_EAM(1, true, true);
+        else if (tab) MDG_EAM_TAB(1, true);
         else MDG_EAM(1, true, false);
     } else {
         if (ev) MDG_EAM(FAST_LPA, false, true);
+        else if (tab) MDG_EAM_TAB(FAST_LPA, false);
         else MDG_EAM(FAST_LPA, false, false);
     }
+#undef MDG_EAM_TAB
 #undef MDG_EAM
     MDG_LAUNCH_CHECK("force_eam");
     return wtime() - t0;
