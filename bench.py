@@ -314,7 +314,7 @@ def bench_eam(args, mod, nx=32, resort=0):
                        "atoms": natoms, "avg_neighbors": sum_nn / nlocal},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "k_eam_density + k_eam_force (4 lanes/atom)",
+                         "kernel": "k_eam_density_tab + k_eam_force_tab (4 lanes/atom, spline rows in LDS)",
                          "avg_launch_ms": avg_ms, "launches": nl.value,
                          "bytes_per_launch": bytes_launch},
         }
