@@ -103,7 +103,10 @@ void energyVirialClusterGFX950(double* epot, double* virial);
    numneigh_masked, neighbors (CSR), cbin_start, cbin_list, atoms_x, atoms_v, atoms_type. */
 long long downloadClusterArrayGFX950(const char* name, void* out, long long cap);
 /* Overwrites the first n values of a named device array from host (test hook; names: slot_x,
-   slot_v).  Returns n, -1 unknown name, -2 n larger than the array. */
+   slot_v).  Returns n, -1 unknown name, -2 n larger than the array.  The lists and the cluster
+   bounding boxes keep their last build's values (the one-type FP32 kernels take positions
+   relative to the i-cluster's box centre, so positions far from their build-time boxes lose
+   FP32 precision until the next rebuild). */
 long long uploadClusterArrayGFX950(const char* name, const void* in, long long n);
 /* nci ncj nghost_clusters dummy_cj nghost_atoms mbins nstencil maxneighs M N natoms */
 void clusterInfoGFX950(int* info);
