@@ -1261,13 +1261,18 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     constexpr int CPG = 16 / N;  // j-clusters per group per iteration
     __shared__ float s_tab[4][MAXT2];
     __shared__ double s_cut[MAXT2];
+    // per type pair, read with one 16-byte LDS load: c12, -c6 and the FP32 band edges
+    __shared__ float4 s_pk[MAXT2];
     if (!UNIFORM) {
         for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
-            s_tab[0][k] = tab.c12[k];
-            s_tab[1][k] = tab.c6[k];
-            s_tab[2][k] = tab.eps4[k];
-            s_tab[3][k] = tab.sig6[k];
-            s_cut[k]    = tab.cut2d[k];
+            s_tab[0][k]  = tab.c12[k];
+            s_tab[1][k]  = tab.c6[k];
+            s_tab[2][k]  = tab.eps4[k];
+            s_tab[3][k]  = tab.sig6[k];
+            s_cut[k]     = tab.cut2d[k];
+            const float c = (float)tab.cut2d[k];
+            s_pk[k]      = make_float4(tab.c12[k], -tab.c6[k], c * (1.0f - CUT_BAND),
+                     c * (1.0f + CUT_BAND));
         }
         __syncthreads();
     }
@@ -1346,16 +1351,17 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             }
             const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
-            double cut2 = cut2u;
+            const int tij = UNIFORM ? 0 : ti[a] * ntypes + tj;
             if (!UNIFORM) {
-                const int tij = ti[a] * ntypes + tj;
-                c12           = s_tab[0][tij];
-                nc6           = -s_tab[1][tij];
-                eps4          = s_tab[2][tij];
-                sig6          = s_tab[3][tij];
-                cut2          = s_cut[tij];
-                clo           = (float)cut2 * (1.0f - CUT_BAND);
-                chi           = (float)cut2 * (1.0f + CUT_BAND);
+                const float4 pk = s_pk[tij];
+                c12             = pk.x;
+                nc6             = pk.y;
+                clo             = pk.z;
+                chi             = pk.w;
+                if (EV) {
+                    eps4 = s_tab[2][tij];
+                    sig6 = s_tab[3][tij];
+                }
             }
             bool in          = rsq < clo;
             const bool below = rsq < chi;
@@ -1378,6 +1384,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                 const double dyd = pyi - J.y;
                 const double dzd = pzi - J.z;
                 const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+                const double cut2 = UNIFORM ? cut2u : s_cut[tij];
                 if (below && !in) in = rsqd < cut2;
             }
             if (SELF || EV) in = in && rsq > 0.0f;  // the self pair is the only one at distance 0
