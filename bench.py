@@ -16,7 +16,8 @@ Workloads (seeded fcc lattices, synthetic velocities):
              vl-lj-64-sort100: the same with the atoms re-sorted by bin every 100 steps
              (MD-Bench's SORT_ATOMS option).
   eam-32     Verlet-list EAM, FP64: 32^3 Cu cells (131,072 atoms), a 3.615 A, synthetic funcfl
-             tables (nr = nrho = 2000, rc 4.95 A), 1600 K, dt 1 fs, reneighbour every 10 steps.
+             tables (nr = nrho = 2000, rc 4.95 A), seeded Maxwell-Boltzmann velocities at
+             1600 K, dt 1 fs, reneighbour every 10 steps.
              eam-64: the same at 64^3 cells (1,048,576 atoms); eam-64-sort100 with the atoms
              re-sorted by bin every 100 steps.
   ka-128     Kob-Andersen 80/20 binary LJ mixture, FP32-mixed cluster pair 4x4: 128^3 cells
@@ -46,6 +47,8 @@ HBM_PEAK_GBS = 8000.0
 # HIP events bracket every EVENT_EVERY-th launch of the reported kernel inside the timed region
 # (each event pair costs ~5 us of idle GPU between kernels, 2 % of a 1M-atom cluster step)
 EVENT_EVERY = 10
+# EAM workloads start from seeded Maxwell-Boltzmann velocities at 1600 K
+MAXWELL_SEED = 1600
 
 
 def binding():
@@ -58,12 +61,28 @@ def binding():
     return mod
 
 
-def host_cores():
+def host_threads():
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except Exception:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def host_cores():
+    """Threads for the CPU baseline: every CPU this process may run on, up to 16 (the CPU share
+    a one-GPU job gets on the GPU hosts, whatever the machine shows)."""
+    return max(1, min(host_threads(), 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
 
 
 def cpu_flags():
@@ -225,10 +244,12 @@ def bench_cp(args, mod, N=4, half=0):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair math, f64 state",
             "data": "synthetic",
             "config": {"workload": "%s: LJ cluster-pair 4x%d %s list, FP32-mixed, 64^3 fcc "
-                                   "cells (%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20"
+                                   "cells (%d atoms), rho 0.8442, rc 2.5, skin 0.3, reneigh 20, plain "
+                                   "(uncompensated) FP32 force sums"
                                    % (wl, N, "half" if half else "full", natoms),
                        "atoms": natoms, "i_clusters": nci,
                        "j_clusters_per_i_cluster": pairs / nci,
+                       "compensated": 0,
                        "energy_per_atom": {"start": e_start, "end": e_end,
                                            "drift": e_end - e_start,
                                            "over_steps": args.steps}},
@@ -269,7 +290,8 @@ def cpu_baseline_eam(nx, steps):
 def bench_eam(args, mod, nx=32, resort=0):
     import ctypes as C
     sim = mod.VLSim(nx=nx, ny=nx, nz=nx, force_field=1, lattice_a=3.615, temp=1600.0,
-                    reneigh_every=10, ntimes=args.steps + args.warmup, resort_every=resort)
+                    maxwell_seed=MAXWELL_SEED, reneigh_every=10,
+                    ntimes=args.steps + args.warmup, resort_every=resort)
     try:
         natoms = sim.atom.Natoms
         sim.run(args.warmup)
@@ -306,7 +328,7 @@ def bench_eam(args, mod, nx=32, resort=0):
             "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "eam-%d%s: Verlet-list EAM FP64, %d^3 Cu fcc cells (%d atoms), "
-                                   "a 3.615 A, synthetic funcfl nr=nrho=2000 rc 4.95 A, 1600 K, "
+                                   "a 3.615 A, synthetic funcfl nr=nrho=2000 rc 4.95 A, Maxwell 1600 K, "
                                    "dt 1 fs, reneigh 10%s"
                                    % (nx, "-sort%d" % resort if resort else "", nx, natoms,
                                       ", atoms re-sorted by bin every %d steps" % resort
@@ -438,7 +460,7 @@ def bench_surrogate(args, mod, nx=64):
     del data
     L = mod.lib()
     sim = mod.VLSim(nx=nx, ny=nx, nz=nx, force_field=1, lattice_a=3.615, temp=1600.0,
-                    reneigh_every=10, ntimes=0)
+                    maxwell_seed=MAXWELL_SEED, reneigh_every=10, ntimes=0)
     try:
         natoms = sim.atom.Natoms
         # plain EAM on this system
@@ -493,7 +515,7 @@ def bench_surrogate(args, mod, nx=64):
             "data": "synthetic",
             "config": {"workload": "eam-surrogate-64: learned EAM surrogate (24 radial + 8 angular "
                                    "descriptors, MLP 32-64-64-1) on %d^3 Cu cells (%d atoms), "
-                                   "1600 K start, plain-EAM judge every 50 steps" % (nx, natoms),
+                                   "Maxwell 1600 K start, plain-EAM judge every 50 steps" % (nx, natoms),
                        "atoms": natoms,
                        "plain_eam_atom_steps_per_s": plain,
                        "speedup_vs_plain_eam": value / plain,
@@ -659,6 +681,9 @@ def main():
     res.setdefault("cpu_baseline", cb)
     if res["cpu_baseline"] is None:
         res["cpu_baseline"] = cb
+    if res["cpu_baseline"]:
+        res["cpu_baseline"]["cpu_model"] = cpu_model()
+        res["cpu_baseline"]["host_threads_visible"] = host_threads()
     print(json.dumps(res, default=float))
 
 

@@ -48,6 +48,11 @@ typedef struct {
        (SORT_ATOMS, src/verletlist/main.c:92-95, 372-381; the reference's default period is
        400); 0 = no sorting, the reference's default build (config.mk:29) */
     int resort_every;
+    /* > 0: initial velocities drawn from the Maxwell-Boltzmann distribution (independent
+       normal components from a generator seeded with this value), then centre-of-mass motion
+       removed and rescaled to temp as adjustThermo does (src/common/thermo.c:102-159);
+       0 = createAtom's uniform Park-Miller velocities (src/verletlist/atom.c:190-205) */
+    int maxwell_seed;
 } mdb_config;
 
 /* Defaults of initParameter (src/common/parameter.c:72-124) with -1/0 "keep default" markers. */
@@ -58,6 +63,9 @@ void mdbDefaultConfig(mdb_config* cfg);
    of up to cap atoms; returns the atom count or a negative error. */
 int mdbCreateLattice(const mdb_config* cfg, double* x, double* v, int* type, int cap);
 double mdbMyrandom(int* seed); /* Park-Miller, src/common/util.c:24-33 */
+/* n x 3 independent standard normal values (Box-Muller on a splitmix64 stream seeded with
+   seed), the velocity draw behind mdb_config.maxwell_seed. */
+void mdbMaxwellNormals(int n, unsigned long long seed, double* v);
 /* funcfl tables -> 7-coefficient splines (src/common/eam_utils.c:42-284), one element. */
 typedef struct {
     int nr, nrho, nr_tot, nrho_tot;

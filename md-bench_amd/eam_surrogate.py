@@ -154,12 +154,13 @@ class Dev:
 # ---------------------------------------------------------------- snapshots
 def capture_frames(temps, per_temp, every=50, nx=10, log=None):
     """Plain-EAM runs (synthetic Cu tables, a = 3.615 A, dt = 1 fs, reneighbour every 10 steps)
-    at each temperature; a snapshot after every `every` steps.  Lists are cut to the descriptor
+    at each temperature from seeded Maxwell-Boltzmann velocities; a snapshot after every `every`
+    steps.  Lists are cut to the descriptor
     cutoff."""
     frames = []
     for T in temps:
         sim = md.VLSim(nx=nx, ny=nx, nz=nx, force_field=1, lattice_a=A_CU, temp=float(T),
-                       reneigh_every=10, ntimes=0)
+                       reneigh_every=10, ntimes=0, maxwell_seed=int(T) + 1)
         try:
             for _ in range(per_temp):
                 sim.run(every)

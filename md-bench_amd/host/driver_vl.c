@@ -158,6 +158,9 @@ int mdbVLCreate(const mdb_config* cfg, mdb_vl** out)
             return MDG_ERR_CAPACITY;
         }
         mdb_setup_thermo(&s->thermo, p, s->atom.Natoms);
+        if (cfg->maxwell_seed > 0)
+            mdbMaxwellNormals(s->atom.Nlocal, (unsigned long long)cfg->maxwell_seed,
+                s->atom.vx);
         mdb_adjust_thermo(&s->thermo, p, &s->atom);
     }
     int n = s->atom.Nlocal;
@@ -210,6 +213,8 @@ int mdbCreateLattice(const mdb_config* cfg, double* x, double* v, int* type, int
     int rc    = mdb_create_atoms(&p, &a);
     if (rc == 0) {
         mdb_setup_thermo(&th, &p, a.Natoms);
+        if (cfg->maxwell_seed > 0)
+            mdbMaxwellNormals(a.Nlocal, (unsigned long long)cfg->maxwell_seed, a.vx);
         mdb_adjust_thermo(&th, &p, &a);
         rc = a.Nlocal;
         if (rc > cap) rc = -3;

@@ -5,6 +5,10 @@
 #include "mdbench_driver.h"
 #include "mdgfx950.h"
 
+/* largest atom type a dump file may hold (the device kernels take up to 8 types and reject
+   more themselves) */
+#define MDB_MAX_TYPES 8
+
 typedef struct {
     double mvv2e, dof_boltz, t_scale, p_scale;
 } mdb_thermo;

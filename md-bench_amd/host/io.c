@@ -47,6 +47,7 @@ int mdbReadDmp(const char* path, int cap, double* x, double* v, int* type, doubl
     if (!fp) return -1;
     char line[4096];
     int natoms = 0, read = 0, ts = -1, nt = 0, rc = 0;
+    unsigned char* seen = NULL; /* one flag per id: a repeated id is an error */
     while (!read && next_line(fp, line, sizeof(line))) {
         if (strncmp(line, "ITEM: ", 6) != 0) {
             rc = -3; /* the reference rejects anything but an item line here */
@@ -75,6 +76,12 @@ int mdbReadDmp(const char* path, int cap, double* x, double* v, int* type, doubl
                 box[2 * d + 1] = tok ? atof(tok) : 0.0;
             }
         } else if (strncmp(item, "ATOMS id type x y z vx vy vz", 28) == 0) {
+            free(seen);
+            seen = (unsigned char*)calloc(natoms > 0 ? natoms : 1, 1);
+            if (!seen) {
+                rc = -5;
+                break;
+            }
             for (int i = 0; i < natoms; i++) {
                 if (!next_line(fp, line, sizeof(line))) {
                     rc = -3;
@@ -89,10 +96,13 @@ int mdbReadDmp(const char* path, int cap, double* x, double* v, int* type, doubl
                     tok  = strtok(NULL, " ");
                     f[k] = tok ? atof(tok) : 0.0;
                 }
-                if (id < 0 || id >= natoms) {
+                /* ids from 1 to natoms, each once; types from 0 (they index the per-type and
+                   type-pair parameter tables) */
+                if (id < 0 || id >= natoms || seen[id] || t < 0 || t >= MDB_MAX_TYPES) {
                     rc = -3;
                     break;
                 }
+                seen[id] = 1;
                 type[id] = t;
                 for (int d = 0; d < 3; d++) {
                     x[3 * id + d] = f[d];
@@ -107,8 +117,9 @@ int mdbReadDmp(const char* path, int cap, double* x, double* v, int* type, doubl
         }
     }
     fclose(fp);
+    free(seen);
     if (rc) return rc;
-    if (ts < 0 || !natoms || !read) return -3;
+    if (ts < 0 || !natoms || read != natoms) return -3;
     if (ntypes) *ntypes = nt;
     return natoms;
 }

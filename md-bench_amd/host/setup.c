@@ -28,6 +28,37 @@ double mdbMyrandom(int* seed)
     return (1.0 / m) * (*seed);
 }
 
+/* splitmix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number generators", OOPSLA
+   2014): 64-bit state, one output per call */
+static unsigned long long splitmix64(unsigned long long* s)
+{
+    unsigned long long z = (*s += 0x9e3779b97f4a7c15ULL);
+    z                    = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z                    = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+/* uniform in (0, 1]: the top 53 bits, shifted off zero so that the logarithm is finite */
+static double unit_open0(unsigned long long* s)
+{
+    return ((double)(splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+void mdbMaxwellNormals(int n, unsigned long long seed, double* v)
+{
+    /* Box-Muller (Box and Muller, Ann. Math. Statist. 29, 1958): two uniforms give two
+       independent standard normals */
+    unsigned long long s = seed;
+    const long long m    = 3LL * n;
+    for (long long k = 0; k < m; k += 2) {
+        const double u1 = unit_open0(&s), u2 = unit_open0(&s);
+        const double r  = sqrt(-2.0 * log(u1));
+        const double t  = 6.283185307179586 * u2;
+        v[k]            = r * cos(t);
+        if (k + 1 < m) v[k + 1] = r * sin(t);
+    }
+}
+
 void mdbDefaultConfig(mdb_config* c)
 {
     memset(c, 0, sizeof(*c));

@@ -33,7 +33,7 @@ class Config(C.Structure):
         ("pair_epsilon", C.POINTER(C.c_double)), ("pair_sigma", C.POINTER(C.c_double)),
         ("pair_cutoff", C.POINTER(C.c_double)), ("type1_fraction", C.c_double),
         ("type_seed", C.c_int), ("input_file", C.c_char_p),
-        ("prune_every", C.c_int), ("resort_every", C.c_int),
+        ("prune_every", C.c_int), ("resort_every", C.c_int), ("maxwell_seed", C.c_int),
     ]
 
 

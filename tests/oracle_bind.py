@@ -42,11 +42,16 @@ _lib = None
 
 def lib():
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("oracle library missing: run `make -C oracle`")
-    L = C.CDLL(LIB_PATH)
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle`")
+        _lib = load(LIB_PATH)
+    return _lib
+
+
+def load(path):
+    """Loads an oracle build (the default one, or a copy built elsewhere) with its signatures."""
+    L = C.CDLL(path)
     L.orc_vl_setup_grid.argtypes = [C.POINTER(VLGrid), C.c_int, C.c_int, C.c_int, C.c_double,
                                     C.c_double]
     L.orc_vl_coord2bin.argtypes = [C.POINTER(VLGrid), C.c_double, C.c_double, C.c_double]
@@ -81,7 +86,6 @@ def lib():
                                  C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp]
     L.orc_vl_sort_atoms.argtypes = [C.POINTER(VLGrid), C.c_int, _dp, _dp, _dp, _ip]
     L.orc_vl_run_resort.argtypes = L.orc_vl_run.argtypes + [C.c_int, _ip]
-    _lib = L
     return L
 
 
@@ -129,12 +133,13 @@ def update_pbc(nlocal, x, bmap, pbc, prd):
     return xx
 
 
-def force_lj(nlocal, x, type_, ntypes, eps, sig6, cutf, numneigh, nb, half):
+def force_lj(nlocal, x, type_, ntypes, eps, sig6, cutf, numneigh, nb, half, L=None):
+    """Oracle LJ forces, pair energy and virial (L: an oracle build other than the default)."""
     nall = x.size // 3
     f = np.zeros(3 * nall, np.float64)
     e = C.c_double()
     w = C.c_double()
-    lib().orc_vl_force_lj(nlocal, nall, np.ascontiguousarray(x, np.float64),
+    (L or lib()).orc_vl_force_lj(nlocal, nall, np.ascontiguousarray(x, np.float64),
                           np.ascontiguousarray(type_, np.int32), ntypes,
                           np.ascontiguousarray(eps, np.float64),
                           np.ascontiguousarray(sig6, np.float64),

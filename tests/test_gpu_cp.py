@@ -141,6 +141,44 @@ def test_prune_matches_reference(variant):
         sim.close()
 
 
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+def test_own_jcluster_is_entry_zero(shape):
+    """The FP32 full-list kernel tests for a pair at distance 0 in its first iteration only: it
+    relies on the j-cluster that holds an i-cluster's own atoms (cj = ci*M/N) being entry 0 of
+    that i-cluster's list whenever the cluster has atoms, and on no other entry being that
+    j-cluster.  Checked after the setup build, after rebuilds on a moving system and after
+    prunes."""
+    M, N = shape
+    P = product()
+    sim = P.CPSim(nx=8, ny=8, nz=8, cluster_m=M, cluster_n=N, precision=0, ntimes=0,
+                  prune_every=7, reneigh_every=20)
+
+    def check(tag):
+        nn = sim.get("numneigh")
+        nb = sim.get("neighbors")
+        nat = sim.get("ci_natoms")
+        off = np.concatenate([[0], np.cumsum(nn)])
+        for ci in range(nn.size):
+            lst = nb[off[ci]:off[ci + 1]]
+            own = ci * M // N
+            if nat[ci] > 0:
+                assert lst.size > 0 and lst[0] == own, (tag, ci, lst[:4])
+                assert np.count_nonzero(lst == own) == 1, (tag, ci)
+            else:
+                assert lst.size == 0, (tag, ci)
+
+    try:
+        check("setup")
+        for k in range(3):
+            sim.run(20)  # each run ends on a rebuild, with prunes at steps 7 and 14 between
+            check("rebuild %d" % k)
+            sim.run(7)
+            check("prune %d" % k)
+            sim.run(13)
+    finally:
+        sim.close()
+
+
 def test_fp64_m1n2_trajectory_with_prune_matches_oracle():
     """Pruning every 3 steps between rebuilds (main.c:329-332 order: before the ghost update):
     the strict 1x2 trajectory equals the oracle's bit for bit."""
@@ -319,12 +357,34 @@ def test_fp32_trajectory_tracks_fp64_and_conserves_energy():
     assert abs((b32 - a32) - (b64 - a64)) <= 0.15 * abs(b64 - a64) + 1e-4
 
 
+def _brute_force_sample(x, L, idx, rc=2.5):
+    """FP64 LJ forces on the atoms idx by direct minimum-image summation over every atom within
+    rc (a periodic k-d tree finds the candidates; no list, bin or ghost code is involved)."""
+    from scipy.spatial import cKDTree
+    xa = np.mod(x.reshape(-1, 3), L)
+    tree = cKDTree(xa, boxsize=L)
+    f = np.zeros((idx.size, 3))
+    for k, i in enumerate(idx):
+        js = np.array(tree.query_ball_point(xa[i], rc), np.int64)
+        js = js[js != i]
+        d = xa[i] - xa[js]
+        d -= L * np.round(d / L)
+        r2 = (d * d).sum(-1)
+        d, r2 = d[r2 < rc * rc], r2[r2 < rc * rc]
+        sr2 = 1.0 / r2
+        sr6 = sr2 ** 3
+        f[k] = (d * (48.0 * sr6 * (sr6 - 0.5) * sr2)[:, None]).sum(0)
+    return f
+
+
 @pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
-def test_full_size_fp32_vs_fp64_kernel(shape):
-    """64^3 cells (1,048,576 atoms): the FP32-mixed kernel against the device FP64 kernel (the
-    latter bit-identical to the oracle on small systems) on the same equilibrated state.  The
-    relative RMS error is measured after 200 steps: right after the lattice start the net forces
-    are small against the pair forces and the relative measure is ill-conditioned."""
+def test_full_size_against_oracle(shape):
+    """64^3 cells (1,048,576 atoms) after 200 steps: the FP32-mixed and the FP64 cluster-pair
+    kernels against the FP64 Verlet-list oracle on the same positions (every atom), and the
+    oracle itself against direct minimum-image sums on 2000 random atoms.  Bars: FP64 forces,
+    energy and virial 1e-12 relative; FP32-mixed forces 1e-6 relative RMS, energy and virial
+    1e-6.  The error is measured on an equilibrated state: right after the lattice start the net
+    forces are small against the pair forces and the relative measure is ill-conditioned."""
     M, N = shape
     P = product()
     sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=1, ntimes=200)
@@ -333,29 +393,49 @@ def test_full_size_fp32_vs_fp64_kernel(shape):
         state = (sim.get("atoms_x"), sim.get("atoms_v"), sim.get("atoms_type"))
     finally:
         sim.close()
-    forces = {}
+    dev = {}
     for prec in (1, 0):
         sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=prec, ntimes=0)
         try:
             sim.set_atoms(*state)
-            forces[prec] = sim.get("slot_f")
             nat = sim.get("ci_natoms")
+            live = _live_slots(nat, M)
+            dev[prec] = (sim.get("slot_x").reshape(-1, 3)[live],
+                         sim.get("slot_f").reshape(-1, 3)[live]) + sim.energy_virial()
         finally:
             sim.close()
-    live = _live_slots(nat, M)
-    a, b = forces[0].reshape(-1, 3)[live], forces[1].reshape(-1, 3)[live]
-    rms = np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
-    print("1M atoms %dx%d: relative RMS force error %.3e, %d atoms" % (M, N, rms, live.size),
-          flush=True)
     assert live.size == 4 * 64 ** 3
-    assert rms <= FP32_RMS_TOL
+    xs = dev[1][0]
+    np.testing.assert_array_equal(dev[0][0], xs)
+    lat = (4 / 0.8442) ** (1 / 3)
+    L = 64 * lat
+    xw = np.mod(xs, L).reshape(-1)
+    one = np.ones(1)
+    fr, e64, w64 = _vl_oracle_forces(xw, np.zeros(live.size, np.int32), 64, lat, one, one,
+                                     one * 6.25, 1, cap_per_atom=120)
+    idx = np.random.default_rng(3).choice(live.size, 2000, replace=False)
+    fb = _brute_force_sample(xw, L, idx)
+    ob_err = np.sqrt(((fr[idx] - fb) ** 2).sum() / (fb ** 2).sum())
+    errs = {}
+    for prec in (1, 0):
+        _, f, e, w = dev[prec]
+        errs[prec] = (np.sqrt(((f - fr) ** 2).sum() / (fr ** 2).sum()), abs(e - e64) / abs(e64),
+                      abs(w - w64) / abs(w64))
+    print("1M atoms %dx%d vs FP64 oracle: FP64 kernel force %.2e energy %.2e virial %.2e; "
+          "FP32-mixed force %.3e energy %.3e virial %.3e; oracle vs direct sums %.2e "
+          "(E/N %.9f, W/N %.9f)" % ((M, N) + errs[1] + errs[0] + (ob_err, e64 / live.size,
+                                                                   w64 / live.size)),
+          flush=True)
+    assert ob_err <= 1e-13
+    assert max(errs[1]) <= 1e-12
+    assert max(errs[0]) <= FP32_RMS_TOL
 
 
 KA = dict(ntypes=2, pair_epsilon=[1.0, 1.5, 1.5, 0.5], pair_sigma=[1.0, 0.8, 0.8, 0.88],
           pair_cutoff=[2.5, 2.0, 2.0, 2.2], rho=1.2, temp=0.8, type1_fraction=0.2, type_seed=4711)
 
 
-def _vl_oracle_forces(x, t, nx, lat, eps, sig6, cutf, ntypes, cutneigh=2.8):
+def _vl_oracle_forces(x, t, nx, lat, eps, sig6, cutf, ntypes, cutneigh=2.8, cap_per_atom=400):
     """FP64 Verlet-list oracle forces (type-pair tables) on atoms x (wrapped into the box)."""
     nl = x.size // 3
     prd = (nx * lat,) * 3
@@ -366,7 +446,7 @@ def _vl_oracle_forces(x, t, nx, lat, eps, sig6, cutf, ntypes, cutneigh=2.8):
     xall = ob.update_pbc(nl, xall, bmap, pbc, prd)
     tall = np.concatenate([t, t[bmap]]).astype(np.int32)
     cn = np.full(ntypes * ntypes, cutneigh ** 2)
-    nn, nb = ob.build_neighbor(g, nl, xall, tall, ntypes, cn, 0)
+    nn, nb = ob.build_neighbor(g, nl, xall, tall, ntypes, cn, 0, cap_per_atom)
     f, e, w = ob.force_lj(nl, xall, tall, ntypes, eps, sig6, cutf, nn, nb, 0)
     return f[:3 * nl].reshape(-1, 3), e, w
 

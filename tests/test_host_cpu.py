@@ -108,3 +108,36 @@ def test_eam_splines_match_reference(case, tmp_path):
     assert t["nr"] == meta(d, "eam_nr")
     for k in ("rhor", "frho", "z2r"):
         np.testing.assert_array_equal(t[k], d["eam_%s_spline" % k])
+
+
+def test_maxwell_velocities():
+    """maxwell_seed: normal velocity components (Box-Muller), then adjustThermo's centre-of-mass
+    removal and rescale.  The draw is reproducible, the components pass a Kolmogorov-Smirnov
+    test against the standard normal and have its fourth moment (3; createAtom's uniform draw
+    has 1.8), and the adjusted set has zero net momentum and exactly the requested
+    temperature."""
+    from scipy import stats
+    P = product()
+    L = P.lib()
+    L.mdbMaxwellNormals.argtypes = [C.c_int, C.c_ulonglong, np.ctypeslib.ndpointer(np.float64)]
+    n = 131072
+    a, b, c = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+    L.mdbMaxwellNormals(n, 4242, a)
+    L.mdbMaxwellNormals(n, 4242, b)
+    L.mdbMaxwellNormals(n, 4243, c)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+    assert stats.kstest(a, "norm").pvalue > 1e-3
+    assert abs(a.mean()) < 5 / np.sqrt(a.size)
+    assert abs(a.var() - 1.0) < 5 * np.sqrt(2.0 / a.size)
+    assert abs(stats.kurtosis(a, fisher=False) - 3.0) < 0.05
+    # through the lattice setup: LJ units, T = 1.44, 8^3 cells
+    x, v, t = P.create_lattice(nx=8, ny=8, nz=8, maxwell_seed=7)
+    x0, v0, _ = P.create_lattice(nx=8, ny=8, nz=8)
+    np.testing.assert_array_equal(x, x0)
+    v = v.reshape(-1, 3)
+    assert np.abs(v.sum(0)).max() < 1e-10
+    temp = (v * v).sum() / (3 * v.shape[0] - 3)
+    assert abs(temp - 1.44) < 1e-12
+    assert abs(stats.kurtosis(v.reshape(-1), fisher=False) - 3.0) < 0.15
+    assert abs(stats.kurtosis(v0.reshape(-1), fisher=False) - 1.8) < 0.05

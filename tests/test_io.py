@@ -49,6 +49,26 @@ def test_reader_rejects_malformed_files(tmp_path):
     import ctypes as C
     rc = P.lib().mdbReadDmp(str(bad).encode(), 10, x, v, t, np.zeros(6), C.byref(C.c_int()))
     assert rc < 0
+    head = "ITEM: TIMESTEP\n0\nITEM: NUMBER OF ATOMS\n3\nITEM: BOX BOUNDS pp pp pp\n" \
+           "0 4\n0 4\n0 4\nITEM: ATOMS id type x y z vx vy vz\n"
+    good = ["1 0 1 1 1 0 0 0", "2 1 2 2 2 0 0 0", "3 0 3 3 3 0 0 0"]
+    cases = {
+        "ok": good,
+        "negative type": ["1 0 1 1 1 0 0 0", "2 -1 2 2 2 0 0 0", "3 0 3 3 3 0 0 0"],
+        "type beyond the tables": ["1 0 1 1 1 0 0 0", "2 8 2 2 2 0 0 0", "3 0 3 3 3 0 0 0"],
+        "repeated id": ["1 0 1 1 1 0 0 0", "1 1 2 2 2 0 0 0", "3 0 3 3 3 0 0 0"],
+        "id beyond the count": ["1 0 1 1 1 0 0 0", "4 1 2 2 2 0 0 0", "3 0 3 3 3 0 0 0"],
+        "missing atom line": good[:2],
+    }
+    for name, lines in cases.items():
+        bad.write_text(head + "\n".join(lines) + "\n")
+        nt = C.c_int()
+        rc = P.lib().mdbReadDmp(str(bad).encode(), 10, x, v, t, np.zeros(6), C.byref(nt))
+        if name == "ok":
+            assert rc == 3 and nt.value == 2, rc
+            assert t[:3].tolist() == [0, 1, 0]
+        else:
+            assert rc < 0, name
 
 
 def _reference_thermo(path, nsteps):
