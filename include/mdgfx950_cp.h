@@ -95,6 +95,17 @@ void finalIntegrateClusterGFX950(Parameter* param, Atom* atom);
 
 /* ------------------------------------------------------------ additions */
 double kineticSumClusterGFX950(Parameter* param, Atom* atom);
+/* No counterpart in the reference.  kineticSumClusterGFX950 for callers that cannot allocate
+   device memory or wait on the device (code under HIP graph capture, tight serving loops): it
+   allocates nothing, copies nothing to the host and does not synchronise; it queues the sum
+   (sum over local atoms of mass * |v|^2, the same kernels and summation order as
+   kineticSumClusterGFX950, so the same value) on the library stream and writes it to d_out[0],
+   device memory the caller owns.  Its 512 partial sums live in a fixed 4 KiB device array that
+   each GPU holds until the process ends (any system size: the kernel strides over the slots);
+   its address is resolved per device in initDeviceClusterGFX950.  Calls that overlap on one
+   GPU share that array and race.  Returns 0, or the status code (MDG_ERR_ARG for a NULL
+   d_out or a device not set up). */
+int kineticSumClusterNoAllocGFX950(Parameter* param, Atom* atom, double* d_out);
 void requestEnergyVirialClusterGFX950(int on);
 void energyVirialClusterGFX950(double* epot, double* virial);
 /* Copies a named device array to host; returns element count, -1 unknown name, -2 too small.

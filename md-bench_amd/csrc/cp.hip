@@ -27,11 +27,23 @@
 #include "mdgfx950_cp.h"
 #include "runtime.h"
 
+extern "C" int mdgfx950Status(void);  // mdgfx950.h (not includable beside the cluster structs)
+
 using namespace mdg;
+
+// Partial sums of kineticSumClusterNoAllocGFX950: a fixed device array, so that entry point
+// allocates nothing.  Every device the library runs on holds its own copy until the process
+// ends.  Its host-side name is not a device address: host code uses only the address that
+// hipGetSymbolAddress returned for the current device (kept per device at
+// initDeviceClusterGFX950), never `scratch` itself.
+constexpr int KSUM_PARTS = 512;
+static __device__ __align__(256) double scratch[KSUM_PARTS];
 
 namespace {
 
 constexpr int BLOCK       = 256;
+constexpr int MAX_DEVICES = 64;
+double* g_scratch[MAX_DEVICES];  // &scratch on each device, resolved once per device
 constexpr int MAXT2       = 64;   // ntypes <= 8
 constexpr int BIN_CAP     = 2048; // atoms per 2D column handled in LDS
 constexpr int JBIN_CAP    = 1024; // j-clusters per column handled in LDS
@@ -1579,18 +1591,42 @@ __global__ void __launch_bounds__(BLOCK) k_cp_ksum(int nslot, int M, const int* 
     }
 }
 
-__global__ void k_cp_sum_partials(int n, double* __restrict__ part)
+// out[0] = the sum of part[0..n) in index order (out may be part)
+__global__ void k_cp_sum_partials(int n, const double* part, double* out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double r = 0;
         for (int k = 0; k < n; k++)
             r += part[k];
-        part[0] = r;
+        out[0] = r;
     }
 }
 
 // ---------------------------------------------------------------- host helpers
 bool ok() { return !failed() && C.ready; }
+
+// The address of `scratch` on the current device, resolved on the first call per device.
+double* scratch_ptr()
+{
+    int dev = 0;
+    MDG_HIP(hipGetDevice(&dev));
+    if (failed()) return nullptr;
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error(E_ARG, "device %d: the scratch table holds devices 0..%d", dev, MAX_DEVICES - 1);
+        return nullptr;
+    }
+    if (!g_scratch[dev]) {
+        void* p      = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    return g_scratch[dev];
+}
 
 void read_status()
 {
@@ -1681,7 +1717,8 @@ void initDeviceClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
             C.uniform = false;
     }
     C.maxneighs = neighbor->maxneighs > 0 ? neighbor->maxneighs : 200;
-    C.ready     = true;
+    if (!scratch_ptr()) return;  // set-up of kineticSumClusterNoAllocGFX950, once per device
+    C.ready = true;
     copyAtomsToClusterGFX950Device(atom);
 }
 
@@ -2286,15 +2323,46 @@ double kineticSumClusterGFX950(Parameter* param, Atom* atom)
     (void)atom;
     if (!ok()) return 0.0;
     TimedRegion tr(F_THERMO);
-    const int nb = 512;
+    const int nb = KSUM_PARTS;
     grow_n(&C.d_red, &C.cap_red, (size_t)nb, 0);
     if (failed()) return 0.0;
     k_cp_ksum<<<nb, BLOCK, 0, stream()>>>(C.nci * C.M, C.M, C.d_ci_natoms, C.d_sv, param->mass, C.d_red);
-    k_cp_sum_partials<<<1, 64, 0, stream()>>>(nb, C.d_red);
+    k_cp_sum_partials<<<1, 64, 0, stream()>>>(nb, C.d_red, C.d_red);
     MDG_LAUNCH_CHECK("cp_ksum");
     MDG_HIP(hipMemcpyAsync(C.h_red, C.d_red, sizeof(double), hipMemcpyDeviceToHost, stream()));
     MDG_HIP(hipStreamSynchronize(stream()));
     return failed() ? 0.0 : C.h_red[0];
+}
+
+int kineticSumClusterNoAllocGFX950(Parameter* param, Atom* atom, double* d_out)
+{
+    (void)atom;
+    if (!ok()) return failed() ? mdgfx950Status() : E_ARG;
+    if (!d_out) {
+        set_error(E_ARG, "kineticSumClusterNoAllocGFX950: no output pointer");
+        return E_ARG;
+    }
+    if ((long long)C.nci * C.M > INT32_MAX) {
+        set_error(E_ARG, "kineticSumClusterNoAllocGFX950: %lld slots exceed its 32-bit indexing",
+            (long long)C.nci * C.M);
+        return E_ARG;
+    }
+    int dev = 0;
+    MDG_HIP(hipGetDevice(&dev));
+    if (failed()) return mdgfx950Status();
+    double* part = (dev >= 0 && dev < MAX_DEVICES) ? g_scratch[dev] : nullptr;
+    if (!part) {
+        set_error(E_ARG, "kineticSumClusterNoAllocGFX950: device %d was not set up by "
+                         "initDeviceClusterGFX950", dev);
+        return E_ARG;
+    }
+    // the same kernels and partial count as kineticSumClusterGFX950, so the same sum; every
+    // partial is written before it is read, so nothing is cleared
+    k_cp_ksum<<<KSUM_PARTS, BLOCK, 0, stream()>>>(C.nci * C.M, C.M, C.d_ci_natoms, C.d_sv,
+        param->mass, part);
+    k_cp_sum_partials<<<1, 64, 0, stream()>>>(KSUM_PARTS, part, d_out);
+    MDG_LAUNCH_CHECK("cp_ksum_noalloc");
+    return mdgfx950Status();
 }
 
 void requestEnergyVirialClusterGFX950(int on) { C.ev_on = on != 0; }

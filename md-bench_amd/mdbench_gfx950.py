@@ -165,6 +165,8 @@ def lib():
     L.computeForceLJClusterGFX950.restype = C.c_double
     L.kineticSumClusterGFX950.argtypes = [vp, vp]
     L.kineticSumClusterGFX950.restype = C.c_double
+    L.kineticSumClusterNoAllocGFX950.argtypes = [vp, vp, vp]
+    L.kineticSumClusterNoAllocGFX950.restype = C.c_int
     L.requestEnergyVirialClusterGFX950.argtypes = [C.c_int]
     L.energyVirialClusterGFX950.argtypes = [_PD, _PD]
     L.downloadClusterArrayGFX950.argtypes = [C.c_char_p, vp, C.c_longlong]
