@@ -50,7 +50,18 @@ constexpr int JBIN_CAP    = 1024; // j-clusters per column handled in LDS
 constexpr double PAD_BASE = 1.0e12, PAD_STEP = 1.0e3;
 constexpr unsigned IMASK_ALL = 0xffffffffu;
 
-enum { ST_BIN_RANGE = 1, ST_NB_OVERFLOW = 2, ST_BIN_CAP = 4, ST_JBIN_CAP = 8 };
+enum { ST_BIN_RANGE = 1, ST_NB_OVERFLOW = 2, ST_BIN_CAP = 4, ST_JBIN_CAP = 8, ST_REC_RANGE = 16 };
+// status word of the record builders (kept across rebuilds, which clear words 0..7; words
+// 12..13 hold the list build's largest cluster height)
+constexpr int ST_REC_WORD = 14;
+
+// Grid of the FP32 j-records (see make_rec): spacing h (a power of two), cell index k =
+// rint(x / h) + bias stored in one byte per axis.
+struct RecGrid {
+    double h, inv_h;
+    float hf;
+    int bias;
+};
 
 struct CGrid {
     double xprd, yprd, zprd;
@@ -97,6 +108,11 @@ struct CPState {
     double* d_sxn   = nullptr;
     size_t cap_sxn  = 0;
     bool sxn_valid  = false;  // padding, dummy and ghost slots of d_sxn match d_sx
+    // FP32 j-records of d_sx and d_sxn (make_rec), for the FP32 full-list kernel
+    float4 *d_sr = nullptr, *d_srn = nullptr;
+    size_t cap_sr = 0, cap_srn = 0;
+    bool rec_valid = false;  // d_sr matches d_sx in every slot
+    RecGrid rg{ 1.0, 1.0, 1.0f, 0 };
     float* d_sf32 = nullptr;
     int* d_st     = nullptr;
     size_t cap_sx = 0, cap_sv = 0, cap_sf64 = 0, cap_sf32 = 0, cap_st = 0;
@@ -158,6 +174,48 @@ __device__ inline unsigned imask_of(int rule, int ci, int cj)
 }
 
 __device__ inline double pad_pos(int slot, int d) { return PAD_BASE + (double)slot * PAD_STEP + d; }
+
+// The FP32 j-record of a slot, one 16-byte load for the FP32 force kernels: x = (k - bias) h + o
+// per axis, with k = rint(x / h) + bias an integer in [0, 255] (one byte of .w each for x, y,
+// z; the atom type in the fourth byte) and o = x - (k - bias) h, exact in FP64 (|o| <= h/2, the
+// subtraction of a nearby multiple of a power of two) and rounded once to FP32 (error at most
+// h 2^-26).  The difference of two cell indices times h is exact in FP32, so a pair
+// displacement built from records carries only the roundings of the offsets and of the final
+// sums.  Far sentinel slots (padding, dummy) keep their FP32 position with k = 0: they stay far
+// from everything.  A position outside the grid is clamped and flags ST_REC_RANGE.
+__device__ inline float4 make_rec(double x, double y, double z, int type, const RecGrid& rg,
+    int* status)
+{
+    const double p[3] = { x, y, z };
+    float o[3];
+    unsigned w = (unsigned)type << 24;
+    if (fabs(x) > 0.5 * PAD_BASE || fabs(y) > 0.5 * PAD_BASE || fabs(z) > 0.5 * PAD_BASE) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            o[d] = (float)p[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double k = rint(p[d] * rg.inv_h) + rg.bias;
+            if (!(k >= 0.0 && k <= 255.0)) {
+                atomicOr(status, ST_REC_RANGE);
+                k = k < 0.0 ? 0.0 : 255.0;
+            }
+            o[d] = (float)(p[d] - (k - rg.bias) * rg.h);
+            w |= (unsigned)k << (8 * d);
+        }
+    }
+    return make_float4(o[0], o[1], o[2], __uint_as_float(w));
+}
+
+// records of every slot from the FP64 positions
+__global__ void k_cp_records(int nslot, const double* __restrict__ sx, const int* __restrict__ st,
+    RecGrid rg, float4* __restrict__ sr, int* __restrict__ status)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    sr[s] = make_rec(sx[3 * s], sx[3 * s + 1], sx[3 * s + 2], st[s], rg, status);
+}
 
 // 26 ghost images in the reference's order (src/clusterpair/pbc.c:248-317): side per axis,
 // 1 = low face (shift +1), 2 = high face (shift -1).
@@ -556,19 +614,25 @@ __global__ void k_cp_ghost_first(int ncj, int nghost, int N, const int* __restri
         cj_bbox[6 * cg + k] = bb[k];
 }
 
-// updatePbcCPU with firstUpdate = 0: one thread per ghost slot coordinate.
+// updatePbcCPU with firstUpdate = 0: one thread per ghost slot; with sr, its FP32 record too
+// (the ghost's own, from its shifted FP64 position).
 __global__ void k_cp_ghost_update(int ncj, int nghost, int N, const int* __restrict__ border,
     const int* __restrict__ pbc, const int* __restrict__ cj_natoms, double xprd, double yprd,
-    double zprd, double* __restrict__ sx)
+    double zprd, double* __restrict__ sx, const int* __restrict__ st, RecGrid rg,
+    float4* __restrict__ sr, int* __restrict__ status)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 3 * nghost * N) return;
-    int d = k % 3, s = k / 3;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nghost * N) return;
     int g = s / N, cjj = s % N;
     if (cjj >= cj_natoms[ncj + g]) return;
-    double prd = d == 0 ? xprd : (d == 1 ? yprd : zprd);
-    int sb     = border[g] * N + cjj;
-    sx[3 * ((ncj + g) * N + cjj) + d] = sx[3 * sb + d] + pbc[3 * g + d] * prd;
+    const int sb = border[g] * N + cjj, sg = (ncj + g) * N + cjj;
+    const double x = sx[3 * sb] + pbc[3 * g] * xprd;
+    const double y = sx[3 * sb + 1] + pbc[3 * g + 1] * yprd;
+    const double z = sx[3 * sb + 2] + pbc[3 * g + 2] * zprd;
+    sx[3 * sg]     = x;
+    sx[3 * sg + 1] = y;
+    sx[3 * sg + 2] = z;
+    if (sr) sr[sg] = make_rec(x, y, z, st[sg], rg, status);
 }
 
 // ---------------------------------------------------------------- binJClusters (neighbor.c:1929-2094)
@@ -1252,25 +1316,37 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
 }
 
 // Full lists, j-atom lanes.  One wavefront works on four i-clusters: the 16 lanes of group g
-// take i-cluster 4w+g, hold its four i-atom positions, and each lane takes one j-atom per
-// iteration (16/N j-clusters of the group's list per iteration) and computes that j-atom's pairs
-// with all four i-atoms.  Every j position is loaded by one lane only and serves four pairs, and
-// each lane keeps the forces of its group's four i-atoms; the 16 lanes of a group reduce them at
-// the end.  The four lists of a wavefront run to the longest; shorter ones read the dummy
-// j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff band and the
-// arithmetic after the displacement are those of k_cp_force_f32.  The one-type variant is held to 80 VGPRs, six wavefronts per
-// SIMD, without spills; the type-table, energy and compensated variants keep their register
-// count (held to 80 they spill inside the loop; held to 96 the type-table one gains nothing).
+// take i-cluster 4w+g, hold its four i-atoms, and each lane takes one j-atom per iteration
+// (16/N j-clusters of the group's list per iteration) and computes that j-atom's pairs with all
+// four i-atoms.  Every j-atom is loaded by one lane only, as one 16-byte record (make_rec), and
+// serves four pairs; each lane keeps the forces of its group's four i-atoms and the 16 lanes of
+// a group reduce them at the end.  The four lists of a wavefront run to the longest; shorter
+// ones read the dummy j-cluster, whose slots sit at far sentinel positions.  Pair set, cutoff
+// band and the arithmetic after the displacement are those of k_cp_force_f32.
+//
+// Displacements from the records:
+// - one-type variants: the j-atom in the frame of the i-cluster's grid cell kI (the cell of its
+//   bounding-box centre), y = (k_j - kI) h + o_j, once per cluster pair (the cell difference
+//   exact, one rounding), the i-atoms the same way once per launch; a pair's displacement
+//   z_a - y: three subtractions per pair;
+// - type-table variants (the Kob-Andersen mixture, whose force error the frame's extra
+//   rounding pushed past 1e-6): d = (k_a - k_j) h + (o_a - o_j) per pair, the cell difference
+//   exact and one rounding at d, nine operations per pair; the j type is the record's fourth
+//   byte.
+// The one-type variant is held to 80 VGPRs, six wavefronts per SIMD, without spills; the
+// type-table, energy and compensated variants keep their register count.
 template <int N, bool EV, bool UNIFORM, bool COMP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((UNIFORM && !COMP && !EV) ? 6 : 1)))
 k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
-    int ntypes, Tab32 tab, const double* __restrict__ sx, const int* __restrict__ st,
-    const int* __restrict__ nb, const int* __restrict__ numneigh,
-    const int* __restrict__ ci_natoms, const double* __restrict__ ci_bbox, int maxneighs,
-    float* __restrict__ f, double* __restrict__ ev, double* __restrict__ v, double dtforce,
-    double* __restrict__ xn, double dt)
+    int ntypes, Tab32 tab, const double* __restrict__ sx, const float4* __restrict__ sr,
+    const int* __restrict__ st, const int* __restrict__ nb, const int* __restrict__ numneigh,
+    const int* __restrict__ ci_natoms, const double* __restrict__ ci_bbox, RecGrid rg,
+    int maxneighs, float* __restrict__ f, double* __restrict__ ev, double* __restrict__ v,
+    double dtforce, double* __restrict__ xn, float4* __restrict__ rn, double dt,
+    int* __restrict__ status)
 {
-    constexpr int CPG = 16 / N;  // j-clusters per group per iteration
+    constexpr int CPG    = 16 / N;  // j-clusters per group per iteration
+    constexpr bool SPLIT = !UNIFORM;
     __shared__ float s_tab[4][MAXT2];
     __shared__ double s_cut[MAXT2];
     // per type pair, read with one 16-byte LDS load: c12, -c6 and the FP32 band edges
@@ -1295,36 +1371,45 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     if (wv * 4 >= nci) return;
     const int lane = threadIdx.x & 63, grp = lane >> 4, sub = lane & 15;
     const int jc = sub / N, jj = sub % N;
-    const unsigned jj24 = 24u * (unsigned)jj;
     const int ci    = wv * 4 + grp;
     const bool live = ci < nci;
-    // One-type variants (SHIFT): positions relative to the centre of the i-cluster's bounding
-    // box (as of the list build; an empty cluster's box is inverted and its centre taken at 0),
-    // the i-atoms once per launch and a j-atom once per cluster pair, each an FP64 difference
-    // rounded to FP32; a pair's displacement is the FP32 difference of the two: three
-    // subtractions per pair instead of three FP64 subtractions and three conversions, ~10 %
-    // fewer vector-ALU cycles.  The two extra roundings raise the 1M-atom LJ force error from
-    // 6.0e-7 to 8.1e-7 (measured), and the Kob-Andersen mixture's from 7.3e-7 to 1.04e-6, over
-    // the 1e-6 bound: the type-table variants keep the rounded FP64 displacement.
-    constexpr bool SHIFT = UNIFORM;
-    using PT             = typename std::conditional<SHIFT, float, double>::type;
-    double ref[3]        = { 0.0, 0.0, 0.0 };
-    if (SHIFT) {
+    // the i-cluster's grid cell: that of the centre of its bounding box (as of the list build;
+    // an empty cluster's box is inverted and its centre taken at 0)
+    float kI[3] = { 0.f, 0.f, 0.f };
+    if (!SPLIT) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double lo = live ? ci_bbox[6 * ci + 2 * d] : 0.0;
             const double hi = live ? ci_bbox[6 * ci + 2 * d + 1] : 0.0;
-            ref[d]          = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+            const double c  = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+            kI[d]           = (float)(rint(c * rg.inv_h) + rg.bias);
         }
     }
-    PT xi[4][3];
+    const float hf = rg.hf;
+    // i-atoms from their own records, by the same operations as a j-atom's, so that the self
+    // pair's displacement is exactly 0: one-type, z = (k_a - kI) h + o_a; type-table, the cell
+    // index and the offset.  Groups past the last i-cluster sit far away (their force sums are
+    // never written; far enough that no pair is inside a cutoff, near enough that r^2 stays
+    // finite, so the energy variant's virial term r^2 * 0 stays 0).
+    float zi[4][3], ki[4][3];
     int ti[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-            xi[a][d] = (PT)((live ? sx[3 * (ci * 4 + a) + d] : 0.0) - ref[d]);
-        ti[a] = (UNIFORM || !live) ? 0 : st[ci * 4 + a];
+        const float4 r   = live ? sr[ci * 4 + a] : make_float4(1e15f, 1e15f, 1e15f, 0.f);
+        const unsigned w = __float_as_uint(r.w);
+        ki[a][0] = (float)(w & 0xffu);
+        ki[a][1] = (float)((w >> 8) & 0xffu);
+        ki[a][2] = (float)((w >> 16) & 0xffu);
+        if (SPLIT) {
+            zi[a][0] = r.x;
+            zi[a][1] = r.y;
+            zi[a][2] = r.z;
+        } else {
+            zi[a][0] = fmaf(ki[a][0] - kI[0], hf, r.x);
+            zi[a][1] = fmaf(ki[a][1] - kI[1], hf, r.y);
+            zi[a][2] = fmaf(ki[a][2] - kI[2], hf, r.z);
+        }
+        ti[a] = UNIFORM ? 0 : (int)(w >> 24);
     }
     const int nn = live ? numneigh[ci] : 0;
     int nmax     = max(nn, __shfl_xor(nn, 16, 64));
@@ -1339,27 +1424,33 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     float fx[4] = {}, fy[4] = {}, fz[4] = {}, cx[4] = {}, cy[4] = {}, cz[4] = {};
     float e = 0.f, w = 0.f;
     struct JL {
-        double x, y, z;
+        float4 r;
         int sj;
     };
     // SELF: the cluster pair may hold pairs at distance 0 (see the list loop below)
     auto use = [&](const JL& J, auto self) {
         constexpr bool SELF = decltype(self)::value;
-        const int tj   = UNIFORM ? 0 : st[J.sj];
-        const float jx = SHIFT ? (float)(J.x - ref[0]) : 0.f,
-                    jy = SHIFT ? (float)(J.y - ref[1]) : 0.f,
-                    jz = SHIFT ? (float)(J.z - ref[2]) : 0.f;
+        const unsigned wj = __float_as_uint(J.r.w);
+        const float kx = (float)(wj & 0xffu), ky = (float)((wj >> 8) & 0xffu),
+                    kz = (float)((wj >> 16) & 0xffu);
+        const int tj = UNIFORM ? 0 : (int)(wj >> 24);
+        float yx = 0.f, yy = 0.f, yz = 0.f;
+        if (!SPLIT) {
+            yx = fmaf(kx - kI[0], hf, J.r.x);
+            yy = fmaf(ky - kI[1], hf, J.r.y);
+            yz = fmaf(kz - kI[2], hf, J.r.z);
+        }
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             float dx, dy, dz;
-            if (SHIFT) {
-                dx = (float)xi[a][0] - jx;
-                dy = (float)xi[a][1] - jy;
-                dz = (float)xi[a][2] - jz;
+            if (SPLIT) {
+                dx = fmaf(ki[a][0] - kx, hf, zi[a][0] - J.r.x);
+                dy = fmaf(ki[a][1] - ky, hf, zi[a][1] - J.r.y);
+                dz = fmaf(ki[a][2] - kz, hf, zi[a][2] - J.r.z);
             } else {
-                dx = (float)((double)xi[a][0] - J.x);
-                dy = (float)((double)xi[a][1] - J.y);
-                dz = (float)((double)xi[a][2] - J.z);
+                dx = zi[a][0] - yx;
+                dy = zi[a][1] - yy;
+                dz = zi[a][2] - yz;
             }
             const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
@@ -1378,23 +1469,20 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             bool in          = rsq < clo;
             const bool below = rsq < chi;
             if (__builtin_expect((__ballot(below) & ~__ballot(in)) != 0, 0)) {
-                // FP64 re-check from the FP64 positions; with SHIFT the i-atom's is read again
-                // (its slot from the wavefront index and the lane id, the latter computed here
-                // by volatile asm, which keeps the compiler from hoisting the address out of the
-                // loop and holding it in a register or spill slot for this rare branch)
-                double pxi = (double)xi[a][0], pyi = (double)xi[a][1], pzi = (double)xi[a][2];
-                if (SHIFT) {
-                    int lr;
-                    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
-                                 : "=v"(lr));
-                    const double* pi = sx + 3 * ((wv * 4 + (lr >> 4)) * 4 + a);
-                    pxi              = pi[0];
-                    pyi              = pi[1];
-                    pzi              = pi[2];
-                }
-                const double dxd = pxi - J.x;
-                const double dyd = pyi - J.y;
-                const double dzd = pzi - J.z;
+                // FP64 re-check from the FP64 positions of both atoms, read again: the i-atom's
+                // slot from the wavefront index and the lane id (computed here by volatile asm,
+                // which keeps the compiler from hoisting the address out of the loop and holding
+                // it in a register or spill slot for this rare branch), clamped to the last
+                // i-cluster for lanes of groups past it
+                int lr;
+                asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
+                             : "=v"(lr));
+                const int cir     = min(wv * 4 + (lr >> 4), nci - 1);
+                const double* pi  = sx + 3 * (cir * 4 + a);
+                const double* pj  = sx + 3 * J.sj;
+                const double dxd  = pi[0] - pj[0];
+                const double dyd  = pi[1] - pj[1];
+                const double dzd  = pi[2] - pj[2];
                 const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
                 const double cut2 = UNIFORM ? cut2u : s_cut[tij];
                 if (below && !in) in = rsqd < cut2;
@@ -1425,8 +1513,8 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     };
     // Entries of the group's list are held 32 at a time in two registers per lane (16 each) and
     // handed out with cross-lane reads inside the group; entries past a list's end read as the
-    // dummy j-cluster.  Positions of the next iteration load while the current ones are used,
-    // and every load in the loop is unconditional (see k_cp_force_f32).
+    // dummy j-cluster.  Records of the next iteration load while the current ones are used, and
+    // every load in the loop is unconditional (see k_cp_force_f32).
     // The list builder puts the entries that need the diagonal mask first, and in a full list
     // the only such entry is the j-cluster holding the i-cluster itself: of a cluster with atoms
     // it is entry 0, and an empty cluster has no entries.  Only the first iteration can meet a
@@ -1435,6 +1523,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     // that of a padding j-slot (sentinels near 1e12 round to multiples of 2^16) gets a NaN
     // force, in accumulators of its own that are never written out; the energy variant, whose
     // sum would take the NaN, keeps the test everywhere.
+    const unsigned jj16 = 16u * (unsigned)jj;
     for (int c0 = 0; c0 < nmax; c0 += 32) {
         const int e0   = c0 + sub < nn ? lst[c0 + sub] : dummy_cj;
         const int e1   = c0 + 16 + sub < nn ? lst[c0 + 16 + sub] : dummy_cj;
@@ -1443,11 +1532,10 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             const int src = (kb & 16) ? e1 : e0;  // kb is wave-uniform, a multiple of CPG
             const int cj  = __builtin_amdgcn_ds_bpermute(((grp << 4) + (kb & 15) + jc) << 2, src);
             J.sj          = cj * N + jj;
-            const double* p = reinterpret_cast<const double*>(
-                reinterpret_cast<const char*>(sx) + (__umul24(cj, 24u * N) + jj24));
-            J.x = p[0];
-            J.y = p[1];
-            J.z = p[2];
+            // 32-bit byte offset from the base, one 24-bit multiply-add (the host keeps every
+            // j-cluster index below 2^24 and every offset below 2^32)
+            J.r = *reinterpret_cast<const float4*>(
+                reinterpret_cast<const char*>(sr) + (__umul24(cj, 16u * N) + jj16));
         };
         JL A, B;
         fetch(c0, A);
@@ -1518,13 +1606,16 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             double vz = v[3 * si + 2] + dtforce * (double)rz;
             if (xn) {
                 // and the next step's initialIntegrate (integrate.c:29-59), which kicks with
-                // the same force: half kick, then the drift into the other position buffer
+                // the same force: half kick, then the drift into the other position buffer,
+                // with its record
                 vx             = vx + dtforce * (double)rx;
                 vy             = vy + dtforce * (double)ry;
                 vz             = vz + dtforce * (double)rz;
-                xn[3 * si]     = px + dt * vx;
-                xn[3 * si + 1] = py + dt * vy;
-                xn[3 * si + 2] = pz + dt * vz;
+                const double nxp = px + dt * vx, nyp = py + dt * vy, nzp = pz + dt * vz;
+                xn[3 * si]     = nxp;
+                xn[3 * si + 1] = nyp;
+                xn[3 * si + 2] = nzp;
+                rn[si]         = make_rec(nxp, nyp, nzp, st[si], rg, status);
             }
             v[3 * si]     = vx;
             v[3 * si + 1] = vy;
@@ -1630,7 +1721,7 @@ double* scratch_ptr()
 
 void read_status()
 {
-    MDG_HIP(hipMemcpyAsync(C.h_status, C.d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MDG_HIP(hipMemcpyAsync(C.h_status, C.d_status, 16 * sizeof(int), hipMemcpyDeviceToHost, stream()));
     MDG_HIP(hipStreamSynchronize(stream()));
 }
 
@@ -1646,6 +1737,7 @@ size_t nslots_all() { return ((size_t)C.ncj + C.nghost + 1) * C.N; }
 bool ensure_slots(size_t nslot, size_t keep_slots)
 {
     C.sxn_valid = false;
+    C.rec_valid = false;
     grow_n(&C.d_sx, &C.cap_sx, 3 * nslot, 3 * keep_slots);
     grow_n(&C.d_st, &C.cap_st, nslot, keep_slots);
     return !failed();
@@ -1717,6 +1809,17 @@ void initDeviceClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor)
             C.uniform = false;
     }
     C.maxneighs = neighbor->maxneighs > 0 ? neighbor->maxneighs : 200;
+    {
+        // record grid: the finest power-of-two spacing whose 256 cells per axis cover the box
+        // plus a margin on both sides (the ghost layer and the drift between rebuilds)
+        const double margin = 2.0 * param->cutneigh + 4.0;
+        const double ext =
+            std::max(param->xprd, std::max(param->yprd, param->zprd)) + 2.0 * margin;
+        double h = 1.0 / 1024.0;
+        while (ext / h > 254.0)
+            h *= 2.0;
+        C.rg = RecGrid{ h, 1.0 / h, (float)h, (int)std::ceil(margin / h) };
+    }
     if (!scratch_ptr()) return;  // set-up of kineticSumClusterNoAllocGFX950, once per device
     C.ready = true;
     copyAtomsToClusterGFX950Device(atom);
@@ -1730,7 +1833,7 @@ void freeDeviceClusterGFX950(void)
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
         C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
-        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn };
+        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn, C.d_sr, C.d_srn };
     for (void* p : ptrs)
         dfree(p);
     if (C.h_status) (void)hipHostFree(C.h_status);
@@ -1885,6 +1988,7 @@ void buildClustersGFX950(Atom* atom)
 {
     if (!ok()) return;
     C.sxn_valid = false;
+    C.rec_valid = false;
     if (C.g.mbins == 0) {
         set_error(E_ARG, "buildClustersGFX950 before setupNeighborClusterGFX950");
         return;
@@ -1916,6 +2020,11 @@ void buildClustersGFX950(Atom* atom)
     if (C.h_status[0] & (ST_BIN_RANGE | ST_BIN_CAP)) {
         set_error(E_BIN, "cluster binning failed (flags %d): an atom outside the grid or a "
                          "column above %d atoms", C.h_status[0], BIN_CAP);
+        return;
+    }
+    if (C.h_status[ST_REC_WORD] & ST_REC_RANGE) {
+        set_error(E_BIN, "FP32 j-records: a position left the record grid (box +- %g)",
+            C.rg.bias * C.rg.h);
         return;
     }
     int nci = read_int(C.d_cstart + mb);
@@ -1954,6 +2063,7 @@ void defineJClustersGFX950(Parameter* param, Atom* atom)
 void setupPbcClusterGFX950(Atom* atom, Parameter* param)
 {
     C.sxn_valid = false;
+    C.rec_valid = false;
     if (!ok()) return;
     TimedRegion tr(F_PBC);
     hipStream_t s = stream();
@@ -1997,13 +2107,17 @@ void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate)
     TimedRegion tr(F_PBC);
     if (firstUpdate) {
         C.sxn_valid = false;
+        C.rec_valid = false;
         k_cp_ghost_first<<<blocks_for(C.nghost + 1, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost,
             C.N, C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx,
             C.d_st, C.d_cj_bbox);
     } else {
-        long long n = 3LL * C.nghost * C.N;
+        // the records of the ghost slots too, while the other records are current (each
+        // record buffer then holds every slot)
+        long long n = (long long)C.nghost * C.N;
         k_cp_ghost_update<<<blocks_for(n, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost, C.N,
-            C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx);
+            C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx,
+            C.d_st, C.rg, C.rec_valid ? C.d_sr : nullptr, C.d_status + ST_REC_WORD);
     }
     MDG_LAUNCH_CHECK("cp_update_pbc");
 }
@@ -2060,6 +2174,7 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     const int nci = C.nci;
     if (neighbor->maxneighs > C.maxneighs) C.maxneighs = neighbor->maxneighs;
     // largest j-cluster height (status words 12-13 hold it as a 64-bit pattern)
+    static_assert(ST_REC_WORD != 12 && ST_REC_WORD != 13, "status words 12..13 hold hmax");
     unsigned long long* d_hmax = reinterpret_cast<unsigned long long*>(C.d_status + 12);
     fill_i32(C.d_status + 12, 0, 2);
     const int ncj_all = C.ncj + C.nghost;
@@ -2176,20 +2291,41 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     const int nci = C.nci, half = neighbor->half_neigh;
     const size_t nloc = (size_t)nci * C.M;
     if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
-    // the next step's positions go to the second buffer, whose padding, dummy and ghost slots
-    // must match the current one (copied once after each rebuild or upload)
-    double* xn = nullptr;
-    if (fuse_initial && fuse_final && C.precision == 0 && !half) {
-        const size_t n3 = 3 * nslots_all(), cap0 = C.cap_sxn;
-        grow_n(&C.d_sxn, &C.cap_sxn, n3, 0);
+    const bool recs = C.precision == 0 && !half;  // the FP32 full-list kernel reads records
+    if (recs) {
+        // the records of every slot, rebuilt from the positions whenever something other than
+        // the integrating force launch and the ghost update has moved them
+        const size_t nsl = nslots_all(), cap0 = C.cap_sr;
+        grow_n(&C.d_sr, &C.cap_sr, nsl, 0);
         if (failed()) return 0;
-        if (C.cap_sxn != cap0) C.sxn_valid = false;
+        if (C.cap_sr != cap0) C.rec_valid = false;
+        if (!C.rec_valid) {
+            k_cp_records<<<blocks_for((long long)nsl, BLOCK), BLOCK, 0, s>>>((int)nsl, C.d_sx,
+                C.d_st, C.rg, C.d_sr, C.d_status + ST_REC_WORD);
+            MDG_LAUNCH_CHECK("cp_records");
+            C.rec_valid = true;
+        }
+    }
+    // the next step's positions and their records go to the second buffers, whose padding,
+    // dummy and ghost slots must match the current ones (copied once after each rebuild or
+    // upload)
+    double* xn = nullptr;
+    float4* rn = nullptr;
+    if (fuse_initial && fuse_final && recs) {
+        const size_t nsl = nslots_all(), n3 = 3 * nsl, cap0 = C.cap_sxn, capr0 = C.cap_srn;
+        grow_n(&C.d_sxn, &C.cap_sxn, n3, 0);
+        grow_n(&C.d_srn, &C.cap_srn, nsl, 0);
+        if (failed()) return 0;
+        if (C.cap_sxn != cap0 || C.cap_srn != capr0) C.sxn_valid = false;
         if (!C.sxn_valid) {
             MDG_HIP(hipMemcpyAsync(C.d_sxn, C.d_sx, n3 * sizeof(double),
+                hipMemcpyDeviceToDevice, s));
+            MDG_HIP(hipMemcpyAsync(C.d_srn, C.d_sr, nsl * sizeof(float4),
                 hipMemcpyDeviceToDevice, s));
             C.sxn_valid = true;
         }
         xn = C.d_sxn;
+        rn = C.d_srn;
     }
     TimedRegion tr(F_FORCE_CP);
     if (C.precision == 0 &&
@@ -2240,9 +2376,9 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
                 C.maxneighs, C.d_sf32, C.d_ev);                                                   \
         else                                                                                     \
             k_cp_force_f32_full<NN, E, U, CO><<<grid4, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
-                C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
-                C.d_ci_bbox, C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn,          \
-                param->dt);                                                                      \
+                C.ntypes, C.t32, C.d_sx, C.d_sr, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,     \
+                C.d_ci_bbox, C.rg, C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn,    \
+                rn, param->dt, C.d_status + ST_REC_WORD);                                        \
     } while (0)
 #define MDG_F32_U(NN, H, E)                                                                      \
     do {                                                                                         \
@@ -2275,6 +2411,8 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     if (fused & FUSED_INITIAL) {
         std::swap(C.d_sx, C.d_sxn);
         std::swap(C.cap_sx, C.cap_sxn);
+        std::swap(C.d_sr, C.d_srn);
+        std::swap(C.cap_sr, C.cap_srn);
     }
     *secs = wtime() - t0;
     return fused;
@@ -2290,6 +2428,7 @@ void initialIntegrateClusterGFX950(Parameter* param, Atom* atom)
         set_error(E_ARG, "initialIntegrateClusterGFX950 before the first force call");
         return;
     }
+    C.rec_valid = false;  // the records follow at the next FP32 force call
     if (C.forces_are_f32)
         k_cp_initial<float><<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.M, C.d_ci_natoms,
             C.d_sx, C.d_sv, C.d_sf32, param->dt, param->dtforce);
@@ -2394,7 +2533,7 @@ long long uploadClusterArrayGFX950(const char* name, const void* in, long long n
     else if (!strcmp(name, "slot_v")) { dst = C.d_sv; avail = 3LL * C.nci * C.M; }
     if (!dst) return -1;
     if (n > avail || n < 0) return -2;
-    if (dst == C.d_sx) C.sxn_valid = false;
+    if (dst == C.d_sx) C.sxn_valid = C.rec_valid = false;
     MDG_HIP(hipMemcpyAsync(dst, in, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream()));
     MDG_HIP(hipStreamSynchronize(stream()));
     return failed() ? -3 : n;

@@ -427,6 +427,7 @@ def test_full_size_against_oracle(shape):
                                                                    w64 / live.size)),
           flush=True)
     assert ob_err <= 1e-13
+    assert np.all(np.isfinite(errs[1] + errs[0]))
     assert max(errs[1]) <= 1e-12
     assert max(errs[0]) <= FP32_RMS_TOL
 
