@@ -1081,6 +1081,15 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f64(int nci, int ncj, int nt
 }
 
 // ---------------------------------------------------------------- FP32-mixed force kernel
+// f(integral_constant<int, I>) for I = B .. E-1 in order, until f returns false
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        if (f(std::integral_constant<int, B>{})) static_for<B + 1, E>(f);
+    }
+}
+
 __device__ inline void kahan_add(float& s, float& c, float t)
 {
     float y  = t - c;
@@ -1425,7 +1434,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     float e = 0.f, w = 0.f;
     struct JL {
         float4 r;
-        int sj;
+        unsigned off;  // byte offset of the record from sr: cj * 16 N + 16 jj
     };
     // SELF: the cluster pair may hold pairs at distance 0 (see the list loop below)
     auto use = [&](const JL& J, auto self) {
@@ -1479,7 +1488,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                              : "=v"(lr));
                 const int cir     = min(wv * 4 + (lr >> 4), nci - 1);
                 const double* pi  = sx + 3 * (cir * 4 + a);
-                const double* pj  = sx + 3 * J.sj;
+                const double* pj  = sx + 3 * (J.off >> 4);  // slot cj * N + jj
                 const double dxd  = pi[0] - pj[0];
                 const double dyd  = pi[1] - pj[1];
                 const double dzd  = pi[2] - pj[2];
@@ -1523,38 +1532,49 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     // that of a padding j-slot (sentinels near 1e12 round to multiples of 2^16) gets a NaN
     // force, in accumulators of its own that are never written out; the energy variant, whose
     // sum would take the NaN, keeps the test everywhere.
+    // Entries are handed out in chunks of 16, one per lane of the group, the next chunk's entry
+    // loaded while the current chunk runs.  Entries are held as the byte offsets of their
+    // j-cluster's records (cj * 16 N; the host keeps every slot offset below 2^32), so a lane
+    // adds only its own record's 16 jj.  Records of the next iteration load while the current
+    // ones are used (across chunk boundaries too), and every load in the loop is unconditional
+    // (see k_cp_force_f32).
     const unsigned jj16 = 16u * (unsigned)jj;
-    for (int c0 = 0; c0 < nmax; c0 += 32) {
-        const int e0   = c0 + sub < nn ? lst[c0 + sub] : dummy_cj;
-        const int e1   = c0 + 16 + sub < nn ? lst[c0 + 16 + sub] : dummy_cj;
-        const int cend = min(nmax, c0 + 32);
-        auto fetch = [&](int kb, JL& J) {
-            const int src = (kb & 16) ? e1 : e0;  // kb is wave-uniform, a multiple of CPG
-            const int cj  = __builtin_amdgcn_ds_bpermute(((grp << 4) + (kb & 15) + jc) << 2, src);
-            J.sj          = cj * N + jj;
-            // 32-bit byte offset from the base, one 24-bit multiply-add (the host keeps every
-            // j-cluster index below 2^24 and every offset below 2^32)
-            J.r = *reinterpret_cast<const float4*>(
-                reinterpret_cast<const char*>(sr) + (__umul24(cj, 16u * N) + jj16));
-        };
-        JL A, B;
-        fetch(c0, A);
-        int k0 = c0;
-        if (c0 == 0) {
-            fetch(CPG, B);
-            use(A, std::true_type{});
-            A  = B;
-            k0 = CPG;
+    const int bp_base   = ((grp << 4) + jc) << 2;
+    auto entry = [&](int k) { return 16u * N * (unsigned)(k < nn ? lst[k] : dummy_cj); };
+    unsigned ec = entry(sub), en = entry(16 + sub);
+    // kb: the wave-uniform index of the next iteration's first entry (a multiple of CPG)
+    auto fetch = [&](int kb, JL& J) {
+        const int kk       = kb & 15;
+        const unsigned src = kk ? ec : en;  // kk == 0: the next chunk's first entry
+        const unsigned cjo = (unsigned)__builtin_amdgcn_ds_bpermute(bp_base + (kk << 2), (int)src);
+        J.off = cjo + jj16;
+        J.r   = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sr) + J.off);
+    };
+    JL A, B;
+    {
+        const unsigned cjo = (unsigned)__builtin_amdgcn_ds_bpermute(bp_base, (int)ec);
+        A.off = cjo + jj16;
+        A.r   = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sr) + A.off);
+    }
+    int k0 = 0;
+    auto advance = [&]() {  // after the iteration at k0, when k0 + CPG starts a new chunk
+        k0 += CPG;
+        if (!(k0 & 15)) {
+            ec = en;
+            en = entry(k0 + 16 + sub);
         }
-        while (k0 < cend) {
-            fetch(k0 + CPG, B);
-            use(A, std::false_type{});
-            k0 += CPG;
-            if (k0 >= cend) break;
-            fetch(k0 + CPG, A);
-            use(B, std::false_type{});
-            k0 += CPG;
-        }
+    };
+    fetch(CPG, B);
+    use(A, std::true_type{});
+    advance();
+    while (k0 < nmax) {
+        fetch(k0 + CPG, A);
+        use(B, std::false_type{});
+        advance();
+        if (k0 >= nmax) break;
+        fetch(k0 + CPG, B);
+        use(A, std::false_type{});
+        advance();
     }
     // reduce over the 16 lanes of each group (lane bits 0..3)
 #pragma unroll
