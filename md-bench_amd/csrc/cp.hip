@@ -125,7 +125,7 @@ struct CPState {
     // j-cluster bins
     int *d_jcnt = nullptr, *d_jstart = nullptr, *d_jlist = nullptr, *d_gcur = nullptr;
     size_t cap_jcnt = 0, cap_jstart = 0, cap_jlist = 0, cap_gcur = 0;
-    double* d_jkey = nullptr;  // bbminz of jlist[m], for the list build's column search
+    double* d_jkey = nullptr;  // bounding box of jlist[m] (6 per entry), in column order
     size_t cap_jkey = 0;
     int* d_zidx = nullptr;     // per column, first jlist position at or above each z boundary
     size_t cap_zidx = 0;
@@ -532,22 +532,40 @@ __global__ void k_cp_define_j(int ncj, int M, int N, const int* __restrict__ ci_
 }
 
 // ---------------------------------------------------------------- setupPbc (pbc.c:188-331)
-__global__ void k_cp_ghost_count(int ncj, const int* __restrict__ cj_natoms,
+// ghost clusters per j-cluster, and the ghost atoms they hold summed into *natoms_ghost: one
+// atomic per block of GC_T threads (same-address atomics serialise at the memory side)
+constexpr int GC_T = 1024;
+__global__ void __launch_bounds__(GC_T) k_cp_ghost_count(int ncj, const int* __restrict__ cj_natoms,
     const double* __restrict__ cj_bbox, double xprd, double yprd, double zprd, double cut,
-    int* __restrict__ cnt)
+    int* __restrict__ cnt, int* __restrict__ natoms_ghost)
 {
-    int cj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cj >= ncj) return;
-    cnt[cj] = cj_natoms[cj] > 0 ? __popc(ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut)) : 0;
+    __shared__ int s_n[GC_T / 64];
+    const int cj = blockIdx.x * blockDim.x + threadIdx.x;
+    int nadd     = 0;
+    if (cj < ncj) {
+        const int na = cj_natoms[cj];
+        const int c  = na > 0 ? __popc(ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut)) : 0;
+        cnt[cj]      = c;
+        nadd         = c * na;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        nadd += __shfl_xor(nadd, o, 64);
+    if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = nadd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < GC_T / 64; w++)
+            t += s_n[w];
+        if (t) atomicAdd(natoms_ghost, t);
+    }
 }
 
 __global__ void k_cp_ghost_fill(int ncj, int N, const int* __restrict__ cj_natoms_in,
     const double* __restrict__ cj_bbox, double xprd, double yprd, double zprd, double cut,
     const int* __restrict__ off, int* __restrict__ border, int* __restrict__ pbc,
-    int* __restrict__ cj_natoms, int* __restrict__ st, int* __restrict__ status)
+    int* __restrict__ cj_natoms, int* __restrict__ st)
 {
     const int cj = blockIdx.x * blockDim.x + threadIdx.x;
-    int nadd     = 0;  // ghost atoms this cluster creates
     if (cj < ncj) {
         const int na = cj_natoms_in[cj];
         if (na > 0) {
@@ -562,15 +580,10 @@ __global__ void k_cp_ghost_fill(int ncj, int N, const int* __restrict__ cj_natom
                 cj_natoms[cg] = na;
                 for (int cjj = 0; cjj < na; cjj++)
                     st[cg * N + cjj] = st[cj * N + cjj];
-                nadd += na;
                 g++;
             }
         }
     }
-    // one atomic per wavefront on the shared counter
-    for (int o = 32; o > 0; o >>= 1)
-        nadd += __shfl_xor(nadd, o, 64);
-    if ((threadIdx.x & 63) == 0 && nadd) atomicAdd(&status[3], nadd);
 }
 
 // updatePbcCPU with firstUpdate = 1: positions, padding and bounding boxes of ghost clusters.
@@ -720,7 +733,9 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
         for (int f = 0; f < n; f++)
             r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
         jlist[out + r] = cj;
-        jkey[out + r]  = k;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+            jkey[6 * (out + r) + q] = cj_bbox[6 * cj + q];
     }
 }
 
@@ -745,7 +760,7 @@ __global__ void k_cp_zindex(int mbins, const int* __restrict__ jstart,
     int lo = jstart[b], hi = jstart[b + 1];
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (jkey[mid] < zb) lo = mid + 1;
+        if (jkey[6 * mid + 4] < zb) lo = mid + 1;
         else hi = mid;
     }
     zidx[t] = lo;
@@ -781,9 +796,9 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 // clusters in column order).  The remaining candidates of up to 64 columns are flattened and
 // tested 64 at a time; ballots give every accepted j-cluster its place in the list.
 template <int M, int N>
-__global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(M == 4 && N == 4 ? 6 : 1))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
-    const double* __restrict__ ci_bbox, const double* __restrict__ cj_bbox,
+    const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
     const int* __restrict__ zidx, double z0, double inv_dz, const double* __restrict__ sx,
     double cutneighsq, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
@@ -858,17 +873,19 @@ __global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fm
             bool pass = false, masked = false;
             int cj    = -1;
             if (t < total) {
+                // the box distance from the box in column order, loaded beside the entry (not
+                // after it) and computed before the half-list test needs the entry
+                const double* jbb = &jbox[6 * m];
+                double d2         = 0.0;
+#pragma unroll
+                for (int d = 2; d >= 0; d--) {
+                    double dl  = ib[2 * d] - jbb[2 * d + 1];
+                    double dh  = jbb[2 * d] - ib[2 * d + 1];
+                    double dm0 = fmax(fmax(dl, dh), 0.0);
+                    d2 += dm0 * dm0;
+                }
                 cj = jlist[m];
                 if (!(half && ci_cj0 > cj)) {
-                    const double* jbb = &cj_bbox[6 * cj];
-                    double d2 = 0.0;
-#pragma unroll
-                    for (int d = 2; d >= 0; d--) {
-                        double dl  = ib[2 * d] - jbb[2 * d + 1];
-                        double dh  = jbb[2 * d] - ib[2 * d + 1];
-                        double dm0 = fmax(fmax(dl, dh), 0.0);
-                        d2 += dm0 * dm0;
-                    }
                     if (d2 < cutneighsq) {
                         pass = d2 < rbb_sq;
                         if (!pass) {
@@ -2033,7 +2050,8 @@ void buildClustersGFX950(Atom* atom)
         C.d_abin_cnt, C.d_abin_atoms);
     k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
         C.d_status);
-    exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb);
+    // the cluster count also to status word 8, read with the flags in one copy
+    exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb, C.d_status + 8);
     MDG_LAUNCH_CHECK("cp_binning");
     read_status();
     if (failed()) return;
@@ -2047,9 +2065,8 @@ void buildClustersGFX950(Atom* atom)
             C.rg.bias * C.rg.h);
         return;
     }
-    int nci = read_int(C.d_cstart + mb);
-    if (failed()) return;
-    C.nci = nci;
+    const int nci = C.h_status[8];
+    C.nci         = nci;
     C.ncj = nci * C.M / C.N;
     grow_n(&C.d_ci_natoms, &C.cap_ci_natoms, (size_t)nci + 1, 0);
     grow_n(&C.d_ci_bin, &C.cap_ci_bin, (size_t)nci + 1, 0);
@@ -2091,10 +2108,11 @@ void setupPbcClusterGFX950(Atom* atom, Parameter* param)
     grow_n(&C.d_gcnt, &C.cap_gcnt, (size_t)ncj + 1, 0);
     grow_n(&C.d_goff, &C.cap_goff, (size_t)ncj + 1, 0);
     if (failed()) return;
-    k_cp_ghost_count<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.d_cj_natoms, C.d_cj_bbox,
-        param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_gcnt);
-    exclusive_scan(C.d_gcnt, C.d_goff, ncj, C.d_status + 2);
+    // ghost clusters (status word 2) and ghost atoms (word 3), read in one copy
     fill_i32(C.d_status + 3, 0, 1);
+    k_cp_ghost_count<<<blocks_for(ncj, GC_T), GC_T, 0, s>>>(ncj, C.d_cj_natoms, C.d_cj_bbox,
+        param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_gcnt, C.d_status + 3);
+    exclusive_scan(C.d_gcnt, C.d_goff, ncj, C.d_status + 2);
     read_status();
     if (failed()) return;
     const int ng = C.h_status[2];
@@ -2109,11 +2127,10 @@ void setupPbcClusterGFX950(Atom* atom, Parameter* param)
     if (failed()) return;
     k_cp_ghost_fill<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.N, C.d_cj_natoms, C.d_cj_bbox,
         param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_goff, C.d_border, C.d_pbc,
-        C.d_cj_natoms, C.d_st, C.d_status);
+        C.d_cj_natoms, C.d_st);
     k_cp_ghost_first<<<blocks_for(ng + 1, BLOCK), BLOCK, 0, s>>>(ncj, ng, C.N, C.d_border, C.d_pbc,
         C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx, C.d_st, C.d_cj_bbox);
     MDG_LAUNCH_CHECK("cp_setup_pbc");
-    read_status();
     C.nghost_atoms        = C.h_status[3];
     atom->Nclusters_ghost = ng;
     atom->dummy_cj        = C.dummy_cj;
@@ -2157,7 +2174,7 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
     grow_n(&C.d_jcnt, &C.cap_jcnt, (size_t)mb + 1, 0);
     grow_n(&C.d_jstart, &C.cap_jstart, (size_t)mb + 1, 0);
     grow_n(&C.d_jlist, &C.cap_jlist, (size_t)C.ncj + ng + 1, 0);
-    grow_n(&C.d_jkey, &C.cap_jkey, (size_t)C.ncj + ng + 1, 0);
+    grow_n(&C.d_jkey, &C.cap_jkey, 6 * ((size_t)C.ncj + ng + 1), 0);
     if (failed()) return;
     fill_i32(C.d_status, 0, 1);
     fill_i32(C.d_gcur, 0, (size_t)mb + 1);
@@ -2210,7 +2227,7 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
         unsigned grid = blocks_for(nci, BLOCK / 64);
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
-        C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_cj_bbox, C.d_jstart,      \
+        C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_jkey, C.d_jstart,         \
         C.d_jlist, C.d_zidx, C.z0, C.inv_dz, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs,  \
         C.d_nb,                                                                                  \
         C.d_numneigh,                                                                            \

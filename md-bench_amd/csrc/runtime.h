@@ -67,7 +67,8 @@ void fill_f64(double* p, double v, size_t n);
 void fill_f32(float* p, float v, size_t n);
 
 // Exclusive scan of n ints into out (in == out allowed); writes the total to *d_total (device).
-void exclusive_scan(const int* in, int* out, int n, int* d_total);
+// out[i] = in[0] + ... + in[i-1]; the total goes to *d_total and, if given, *d_total2.
+void exclusive_scan(const int* in, int* out, int n, int* d_total, int* d_total2 = nullptr);
 
 // Grid size for n items at `block` threads per block.
 inline unsigned blocks_for(long long n, int block)

@@ -217,7 +217,8 @@ __global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __restrict__ d_total)
+__global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __restrict__ d_total,
+    int* __restrict__ d_total2)
 {
     __shared__ int sh[33];
     // each thread owns a contiguous chunk of tile sums
@@ -235,6 +236,7 @@ __global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __res
             off += t;
         }
     if (threadIdx.x == 0 && d_total) *d_total = tot;
+    if (threadIdx.x == 0 && d_total2) *d_total2 = tot;
 }
 
 __global__ void k_scan_apply(const int* __restrict__ in, int* __restrict__ out, int n,
@@ -258,18 +260,50 @@ __global__ void k_scan_apply(const int* __restrict__ in, int* __restrict__ out, 
     }
 }
 
+// The whole scan in one block of SCAN1_T threads, each owning a contiguous run of at most
+// SCAN1_I values (short arrays: one launch instead of three).  in and out may alias.
+constexpr int SCAN1_T = 1024, SCAN1_I = 16;
+__global__ void __launch_bounds__(SCAN1_T) k_scan_single(const int* in, int* out, int n,
+    int* __restrict__ d_total, int* __restrict__ d_total2)
+{
+    __shared__ int sh[33];
+    const int per = (n + SCAN1_T - 1) / SCAN1_T;
+    const int b   = threadIdx.x * per;
+    int v[SCAN1_I];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_I; k++) {
+        v[k] = (k < per && b + k < n) ? in[b + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int off = block_excl_scan(s, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < SCAN1_I; k++) {
+        if (k < per && b + k < n) out[b + k] = off;
+        off += v[k];
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = tot;
+    if (threadIdx.x == 0 && d_total2) *d_total2 = tot;
+}
+
 static int* g_tiles       = nullptr;
 static size_t g_tiles_cap = 0;
 
-void exclusive_scan(const int* in, int* out, int n, int* d_total)
+void exclusive_scan(const int* in, int* out, int n, int* d_total, int* d_total2)
 {
     if (failed()) return;
+    if (n <= SCAN1_T * SCAN1_I) {
+        k_scan_single<<<1, SCAN1_T, 0, stream()>>>(in, out, n, d_total, d_total2);
+        MDG_LAUNCH_CHECK("exclusive_scan");
+        return;
+    }
     int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (ntiles < 1) ntiles = 1;
     if (!grow_n(&g_tiles, &g_tiles_cap, (size_t)ntiles, 0)) return;
     hipStream_t s = stream();
     k_scan_reduce<<<ntiles, SCAN_T, 0, s>>>(in, n, g_tiles);
-    k_scan_tiles<<<1, 1024, 0, s>>>(g_tiles, ntiles, d_total);
+    k_scan_tiles<<<1, 1024, 0, s>>>(g_tiles, ntiles, d_total, d_total2);
     k_scan_apply<<<ntiles, SCAN_T, 0, s>>>(in, out, n, g_tiles);
     MDG_LAUNCH_CHECK("exclusive_scan");
 }
