@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libmdgfx950.so")
+# MDGFX950_LIB names another build of the same library (the sanitizer build of its host code)
+LIB_PATH = os.environ.get("MDGFX950_LIB") or os.path.join(HERE, "lib", "libmdgfx950.so")
 
 FF_LJ, FF_EAM = 0, 1
 

@@ -5,7 +5,8 @@ import os
 import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(REPO, "oracle", "liboracle_md.so")
+# ORACLE_MD_LIB names another build of the oracle (its sanitizer build)
+LIB_PATH = os.environ.get("ORACLE_MD_LIB") or os.path.join(REPO, "oracle", "liboracle_md.so")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
