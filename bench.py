@@ -506,12 +506,17 @@ def bench_surrogate(args, mod, nx=64):
         d = xr[st["neighbors"][:nn.sum()]] - xr[ii]
         pairs = int(((d * d).sum(1) < es.RC ** 2).sum())
         # Algorithmic bytes of one surrogate force call (descriptor, network and force passes
-        # together): per listed neighbour the index and x_j twice (56 B); per pair inside rc the
-        # compact-list entry written and read (8 B) and the neighbour's FP16 coefficient row
-        # (128 B); per atom x_i twice, 32 descriptors and 36 moments written and read, the
-        # coefficient row written and read, the force written (x 4-byte/8-byte as stored).
-        per_atom = 48 + 2 * 4 * (32 + 36) + 2 * 128 + 24
-        bytes_launch = nn.sum() * 56.0 + pairs * (8.0 + 128.0) + nlocal * per_atom
+        # together), the per-unit figures of the gather-bound model for this workload: per
+        # listed neighbour one gather of its index and FP32 position (16 B, one pass over the
+        # list); per atom its position (24 B), 32 descriptors and 36 moments written and read
+        # back (2 x 272 B), its coefficient row written and read (2 x 128 B) and its force
+        # (24 B).  The implementation reads more than this (the list and positions in both
+        # passes, FP64 positions, a neighbour's coefficient row per pair inside rc); that
+        # excess is what keeps it below the bound.
+        per_atom = 24 + 2 * 4 * (32 + 36) + 2 * 128 + 24
+        bytes_launch = nn.sum() * 16.0 + nlocal * per_atom
+        impl_bytes = (nn.sum() * 56.0 + pairs * (8.0 + 128.0)
+                      + nlocal * (48 + 2 * 4 * (32 + 36) + 2 * 128 + 24))
         regions = max(nl.value, 1)
         avg_ms = 3.0 * ms.value / regions  # three timed launches per force call
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
@@ -533,7 +538,9 @@ def bench_surrogate(args, mod, nx=64):
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                          "kernel": "k_sur_desc + k_sur_mlp + k_sur_force",
                          "avg_launch_ms": avg_ms, "launches": regions // 3,
-                         "bytes_per_launch": bytes_launch},
+                         "bytes_per_launch": bytes_launch,
+                         "implementation_bytes_per_launch": impl_bytes,
+                         "pairs_inside_rc": pairs, "listed_neighbours": int(nn.sum())},
         }
         if not args.no_cpu_baseline:
             res["cpu_baseline"] = cpu_baseline_surrogate(model, frame)

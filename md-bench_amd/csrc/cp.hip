@@ -112,6 +112,17 @@ struct CPState {
     float4 *d_sr = nullptr, *d_srn = nullptr;
     size_t cap_sr = 0, cap_srn = 0;
     bool rec_valid = false;  // d_sr matches d_sx in every slot
+    // half lists without atomics: tile tables and the records' reverse map, built from the
+    // current list on the first FP32 force call after a build or prune (tiles_valid); tiles_ok
+    // false when a tile outgrew its tables (that list then runs the atomic kernel)
+    bool tiles_valid = false, tiles_ok = false;
+    int ntiles = 0, tile_ucap = 0;
+    int *d_tile_cj = nullptr, *d_tile_n = nullptr, *d_eslot = nullptr, *d_rec_off = nullptr,
+        *d_rcount = nullptr, *d_rstart = nullptr, *d_rcur = nullptr, *d_rev = nullptr;
+    size_t cap_tile_cj = 0, cap_tile_n = 0, cap_eslot = 0, cap_rec_off = 0, cap_rcount = 0,
+           cap_rstart = 0, cap_rcur = 0, cap_rev = 0;
+    float *d_rec = nullptr, *d_fown = nullptr;
+    size_t cap_rec = 0, cap_fown = 0;
     RecGrid rg{ 1.0, 1.0, 1.0f, 0 };
     float* d_sf32 = nullptr;
     int* d_st     = nullptr;
@@ -1673,6 +1684,495 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     (void)ncj;
 }
 
+// ---------------------------------------------------------------- half lists without atomics
+// A tile is TILE_CI consecutive i-clusters.  Its table lists, in increasing order, the distinct
+// local j-clusters its lists name (at most TILE_UCAP j-atom slots' worth), and every list entry
+// gets its slot in that table (-1 for a ghost j-cluster: in a half list a pair with a ghost acts
+// on its local atom only, its mirror pair sits in the ghost owner's list).  The force pass
+// evaluates every listed cluster pair once: i-forces in registers, j-forces summed in the
+// tile's table in LDS and written once per tile as records; a gather pass then gives each atom
+// its i-force plus the records of its j-cluster, in record order.  The table sums 64-bit fixed
+// point (2^-40 units, range +-2^23): integer adds are exact, so the sums do not depend on the
+// order in which the wavefronts of a tile add (and LDS integer atomics run at full rate, FP32
+// ones did not: 0.57 against 0.21 ms per 1M-atom call).  No global atomics anywhere.
+constexpr int TILE_CI    = 32;    // i-clusters per tile
+constexpr int TILE_ECAP  = 4096;  // list entries of one tile sorted in LDS
+constexpr int TILE_W64   = 7680;  // table capacity in 64-bit words (60 KB): 640 slots of 4x3,
+                                  // 320 of 8x3
+constexpr int TILE_WAVES = 8;     // wavefronts per tile block, four i-clusters each per round
+constexpr double FIX_ONE = 1099511627776.0;  // 2^40, one force unit in table units
+static_assert(TILE_CI % (4 * TILE_WAVES) == 0, "whole rounds per tile");
+
+__global__ void __launch_bounds__(1024) k_cp_tile_table(int nci, int ncj, int ucap,
+    const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
+    int* __restrict__ tile_cj, int* __restrict__ tile_n, int* __restrict__ entry_slot,
+    int* __restrict__ status)
+{
+    __shared__ int s_key[TILE_ECAP];
+    __shared__ int s_uni[TILE_ECAP];
+    static_assert(TILE_CI <= 64, "the list-length prefix is one wavefront scan");
+    __shared__ int s_pre[TILE_CI + 1];
+    __shared__ int s_sh[33];
+    const int t = blockIdx.x, c0 = t * TILE_CI, nc = min(nci - c0, TILE_CI);
+    const int tid = threadIdx.x;
+    // entries of the tile: prefix of the list lengths
+    if (tid < 64) {
+        int v = tid < nc ? numneigh[c0 + tid] : 0;
+        int inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int u = __shfl_up(inc, o, 64);
+            if (tid >= o) inc += u;
+        }
+        if (tid < TILE_CI) s_pre[tid + 1] = inc;
+        if (tid == 0) s_pre[0] = 0;
+    }
+    __syncthreads();
+    const int E = s_pre[TILE_CI];
+    if (E > TILE_ECAP) {
+        if (tid == 0) atomicOr(status, 1);
+        return;
+    }
+    for (int e = tid; e < TILE_ECAP; e += blockDim.x) {
+        int key = INT_MAX;
+        if (e < E) {
+            int c = 0;
+            for (int step = TILE_CI / 2; step > 0; step >>= 1)
+                if (s_pre[c + step] <= e) c += step;
+            const int cj = nb[(size_t)(c0 + c) * maxneighs + (e - s_pre[c])];
+            key = cj < ncj ? cj : INT_MAX;
+        }
+        s_key[e] = key;
+    }
+    __syncthreads();
+    // bitonic sort, ascending
+    for (int k = 2; k <= TILE_ECAP; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < TILE_ECAP; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = s_key[i], b = s_key[l];
+                    if (((i & k) == 0) == (a > b)) {
+                        s_key[i] = b;
+                        s_key[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // distinct keys: flags, a block scan over runs of four per thread, compaction
+    int fl[4], sum = 0;
+    for (int q = 0; q < 4; q++) {
+        const int i = tid * 4 + q;
+        const int k = s_key[i];
+        fl[q]       = (k != INT_MAX && (i == 0 || s_key[i - 1] != k)) ? 1 : 0;
+        sum += fl[q];
+    }
+    int tot;
+    {
+        const int lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+        int inc = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            int u = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) s_sh[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            int v  = lane < nw ? s_sh[lane] : 0;
+            int vi = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                int u = __shfl_up(vi, o, 64);
+                if (lane >= o) vi += u;
+            }
+            if (lane < nw) s_sh[lane] = vi - v;
+            if (lane == nw - 1) s_sh[32] = vi;
+        }
+        __syncthreads();
+        int off = inc - sum + s_sh[w];
+        tot     = s_sh[32];
+        for (int q = 0; q < 4; q++) {
+            if (fl[q]) s_uni[off] = s_key[tid * 4 + q];
+            off += fl[q];
+        }
+    }
+    __syncthreads();
+    if (tot > ucap) {
+        if (tid == 0) atomicOr(status, 2);
+        return;
+    }
+    for (int u = tid; u < tot; u += blockDim.x)
+        tile_cj[(size_t)t * ucap + u] = s_uni[u];
+    if (tid == 0) tile_n[t] = tot;
+    // every entry's slot: its j-cluster's place in the table (binary search)
+    for (int e = tid; e < E; e += blockDim.x) {
+        int c = 0;
+        for (int step = TILE_CI / 2; step > 0; step >>= 1)
+            if (s_pre[c + step] <= e) c += step;
+        const size_t at = (size_t)(c0 + c) * maxneighs + (e - s_pre[c]);
+        const int cj    = nb[at];
+        int slot        = -1;
+        if (cj < ncj) {
+            int lo = 0, hi = tot - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_uni[mid] < cj) lo = mid + 1;
+                else hi = mid;
+            }
+            slot = lo;
+        }
+        entry_slot[at] = slot;
+    }
+}
+
+// records of every local j-cluster: count, then (after a scan) fill and sort each run so the
+// gather visits a cluster's records in increasing record order
+__global__ void k_cp_rev_count(int ntiles, int ucap, const int* __restrict__ tile_cj,
+    const int* __restrict__ tile_n, int* __restrict__ rcount)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(k / ucap), u = (int)(k % ucap);
+    if (t >= ntiles || u >= tile_n[t]) return;
+    atomicAdd(&rcount[tile_cj[(size_t)t * ucap + u]], 1);
+}
+
+__global__ void k_cp_rev_fill(int ntiles, int ucap, const int* __restrict__ tile_cj,
+    const int* __restrict__ tile_n, const int* __restrict__ rec_off,
+    const int* __restrict__ rstart, int* __restrict__ rcur, int* __restrict__ rev)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(k / ucap), u = (int)(k % ucap);
+    if (t >= ntiles || u >= tile_n[t]) return;
+    const int cj = tile_cj[(size_t)t * ucap + u];
+    rev[rstart[cj] + atomicAdd(&rcur[cj], 1)] = rec_off[t] + u;
+}
+
+__global__ void k_cp_rev_sort(int ncj, const int* __restrict__ rstart, int* __restrict__ rev)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncj) return;
+    const int b = rstart[c], e = rstart[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        const int v = rev[i];
+        int j       = i - 1;
+        while (j >= b && rev[j] > v) {
+            rev[j + 1] = rev[j];
+            j--;
+        }
+        rev[j + 1] = v;
+    }
+}
+
+// Half-list force pass over one tile per block (TILE_WAVES wavefronts, each taking four
+// i-clusters per round).  The lane work is that of k_cp_force_f32_full (one j-atom per lane
+// against the four i-atoms of its group, records, cutoff band, FP32 pair math), with the half
+// list's pair rule in the first iteration (entry 0 is the i-cluster's own j-cluster: its pairs
+// with s_j > s_i only) and the j-atom's force summed over its four pairs and added to the
+// tile's table in LDS.  A pair with a ghost j-cluster acts on the i-atom only and, in the
+// energy sums, weighs one half.
+template <int N, bool EV, bool UNIFORM>
+__global__ void __launch_bounds__(TILE_WAVES * 64) k_cp_force_half_tiled(int nci, int ncj,
+    int dummy_cj, int ntypes, Tab32 tab, const double* __restrict__ sx,
+    const float4* __restrict__ sr, const int* __restrict__ nb, const int* __restrict__ slots,
+    const int* __restrict__ numneigh, const double* __restrict__ ci_bbox, RecGrid rg,
+    int maxneighs, int ucap, const int* __restrict__ tile_n, const int* __restrict__ rec_off,
+    float* __restrict__ rec, float* __restrict__ fown, double* __restrict__ ev)
+{
+    constexpr int CPG    = 16 / N;
+    constexpr bool SPLIT = !UNIFORM;
+    constexpr int RF     = 3 * N;  // 64-bit words per table slot
+    extern __shared__ unsigned long long s_table[];
+    __shared__ float s_tab[2][MAXT2];
+    __shared__ double s_cut[MAXT2];
+    __shared__ float4 s_pk[MAXT2];
+    const int t = blockIdx.x;
+    const int ntab = tile_n[t];
+    for (int k = threadIdx.x; k < ntab * RF; k += blockDim.x)
+        s_table[k] = 0ull;
+    if (!UNIFORM) {
+        for (int k = threadIdx.x; k < ntypes * ntypes; k += blockDim.x) {
+            s_tab[0][k]   = tab.eps4[k];
+            s_tab[1][k]   = tab.sig6[k];
+            s_cut[k]      = tab.cut2d[k];
+            const float c = (float)tab.cut2d[k];
+            s_pk[k] = make_float4(tab.c12[k], -tab.c6[k], c * (1.0f - CUT_BAND), c * (1.0f + CUT_BAND));
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, grp = lane >> 4, sub = lane & 15;
+    const int jc = sub / N, jj = sub % N;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const float c12u = tab.c12[0], eps4u = tab.eps4[0], sig6u = tab.sig6[0];
+    float nc6u = -tab.c6[0];
+    asm volatile("" : "+v"(nc6u));
+    const double cut2u = tab.cut2d[0];
+    const float clo_u  = (float)cut2u * (1.0f - CUT_BAND), chi_u = (float)cut2u * (1.0f + CUT_BAND);
+    const float hf     = rg.hf;
+    const unsigned jj16 = 16u * (unsigned)jj;
+    const int bp_base   = ((grp << 4) + jc) << 2;
+    float e = 0.f, w = 0.f;
+    for (int round = 0; round < TILE_CI / (4 * TILE_WAVES); round++) {
+        // the four i-clusters of this wavefront and round, a quarter tile apart: groups of one
+        // wavefront then rarely meet the same j-atom in one iteration, whose LDS adds to one
+        // address would serialise
+        const int base  = __builtin_amdgcn_readfirstlane(t * TILE_CI + round * TILE_WAVES + wave);
+        if (base >= nci) break;
+        const int ci    = base + grp * (TILE_CI / 4);
+        const bool live = ci < nci;
+        const int ci_cj0 = (N == 4) ? ci : ci >> 1;
+        float kI[3] = { 0.f, 0.f, 0.f };
+        if (!SPLIT) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double lo = live ? ci_bbox[6 * ci + 2 * d] : 0.0;
+                const double hi = live ? ci_bbox[6 * ci + 2 * d + 1] : 0.0;
+                const double c  = lo <= hi ? 0.5 * (lo + hi) : 0.0;
+                kI[d]           = (float)(rint(c * rg.inv_h) + rg.bias);
+            }
+        }
+        float zi[4][3], ki[4][3];
+        int ti[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float4 r   = live ? sr[ci * 4 + a] : make_float4(1e15f, 1e15f, 1e15f, 0.f);
+            const unsigned u = __float_as_uint(r.w);
+            ki[a][0] = (float)(u & 0xffu);
+            ki[a][1] = (float)((u >> 8) & 0xffu);
+            ki[a][2] = (float)((u >> 16) & 0xffu);
+            if (SPLIT) {
+                zi[a][0] = r.x;
+                zi[a][1] = r.y;
+                zi[a][2] = r.z;
+            } else {
+                zi[a][0] = fmaf(ki[a][0] - kI[0], hf, r.x);
+                zi[a][1] = fmaf(ki[a][1] - kI[1], hf, r.y);
+                zi[a][2] = fmaf(ki[a][2] - kI[2], hf, r.z);
+            }
+            ti[a] = UNIFORM ? 0 : (int)(u >> 24);
+        }
+        const int nn = live ? numneigh[ci] : 0;
+        int nmax     = max(nn, __shfl_xor(nn, 16, 64));
+        nmax         = max(nmax, __shfl_xor(nmax, 32, 64));
+        nmax         = __builtin_amdgcn_readfirstlane(nmax);
+        const size_t lbase = (size_t)(live ? ci : 0) * maxneighs;
+        float fx[4] = {}, fy[4] = {}, fz[4] = {};
+        struct JL {
+            float4 r;
+            unsigned off;
+            int slot;
+        };
+        auto use = [&](const JL& J, bool first) {
+            const unsigned wj = __float_as_uint(J.r.w);
+            const float kx = (float)(wj & 0xffu), ky = (float)((wj >> 8) & 0xffu),
+                        kz = (float)((wj >> 16) & 0xffu);
+            const int tj   = UNIFORM ? 0 : (int)(wj >> 24);
+            const int sj   = (int)(J.off >> 4);
+            const int cj   = sj / N;
+            float yx = 0.f, yy = 0.f, yz = 0.f;
+            if (!SPLIT) {
+                yx = fmaf(kx - kI[0], hf, J.r.x);
+                yy = fmaf(ky - kI[1], hf, J.r.y);
+                yz = fmaf(kz - kI[2], hf, J.r.z);
+            }
+            float gx = 0.f, gy = 0.f, gz = 0.f;  // the j-atom's force, over its four pairs
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                float dx, dy, dz;
+                if (SPLIT) {
+                    dx = fmaf(ki[a][0] - kx, hf, zi[a][0] - J.r.x);
+                    dy = fmaf(ki[a][1] - ky, hf, zi[a][1] - J.r.y);
+                    dz = fmaf(ki[a][2] - kz, hf, zi[a][2] - J.r.z);
+                } else {
+                    dx = zi[a][0] - yx;
+                    dy = zi[a][1] - yy;
+                    dz = zi[a][2] - yz;
+                }
+                const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                float c12 = c12u, nc6 = nc6u, eps4 = eps4u, sig6 = sig6u, clo = clo_u, chi = chi_u;
+                const int tij = UNIFORM ? 0 : ti[a] * ntypes + tj;
+                if (!UNIFORM) {
+                    const float4 pk = s_pk[tij];
+                    c12             = pk.x;
+                    nc6             = pk.y;
+                    clo             = pk.z;
+                    chi             = pk.w;
+                    if (EV) {
+                        eps4 = s_tab[0][tij];
+                        sig6 = s_tab[1][tij];
+                    }
+                }
+                bool in          = rsq < clo;
+                const bool below = rsq < chi;
+                if (__builtin_expect((__ballot(below) & ~__ballot(in)) != 0, 0)) {
+                    int lr;
+                    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
+                                 : "=v"(lr));
+                    const int cir     = min(base + (lr >> 4) * (TILE_CI / 4), nci - 1);
+                    const double* pi  = sx + 3 * (cir * 4 + a);
+                    const double* pj  = sx + 3 * sj;
+                    const double dxd  = pi[0] - pj[0];
+                    const double dyd  = pi[1] - pj[1];
+                    const double dzd  = pi[2] - pj[2];
+                    const double rsqd = dxd * dxd + dyd * dyd + dzd * dzd;
+                    const double cut2 = UNIFORM ? cut2u : s_cut[tij];
+                    if (below && !in) in = rsqd < cut2;
+                }
+                // half-list rule in the i-cluster's own j-cluster (entry 0): s_j > s_i only; the
+                // energy variant also drops pairs at distance 0 everywhere (padding slots whose
+                // FP32 sentinels coincide), which would put a NaN into its sums
+                if (first) in = in && !(cj == ci_cj0 && !(ci * 4 + a < sj));
+                if (EV) in = in && rsq > 0.0f;
+                const float r2i = __builtin_amdgcn_rcpf(rsq);
+                const float r6i = r2i * r2i * r2i;
+                float fpair     = r6i * fmaf(c12, r6i, nc6) * r2i;
+                fpair           = in ? fpair : 0.0f;
+                fx[a] = fmaf(dx, fpair, fx[a]);
+                fy[a] = fmaf(dy, fpair, fy[a]);
+                fz[a] = fmaf(dz, fpair, fz[a]);
+                gx    = fmaf(-dx, fpair, gx);
+                gy    = fmaf(-dy, fpair, gy);
+                gz    = fmaf(-dz, fpair, gz);
+                if (EV) {
+                    const float sr6 = r6i * sig6;
+                    const float wgt = cj < ncj ? 1.0f : 0.5f;
+                    e += in ? wgt * eps4 * sr6 * (sr6 - 1.0f) : 0.0f;
+                    w += wgt * rsq * fpair;
+                }
+            }
+            if (J.slot >= 0) {
+                unsigned long long* q = s_table + (J.slot * N + jj) * 3;
+                atomicAdd(q, (unsigned long long)__double2ll_rn((double)gx * FIX_ONE));
+                atomicAdd(q + 1, (unsigned long long)__double2ll_rn((double)gy * FIX_ONE));
+                atomicAdd(q + 2, (unsigned long long)__double2ll_rn((double)gz * FIX_ONE));
+            }
+        };
+        auto entry = [&](int k) {
+            return k < nn ? make_int2((int)(16u * N * (unsigned)nb[lbase + k]), slots[lbase + k])
+                          : make_int2((int)(16u * N * (unsigned)dummy_cj), -1);
+        };
+        int2 ec = entry(sub), en = entry(16 + sub);
+        auto fetch = [&](int kb, JL& J) {
+            const int kk    = kb & 15;
+            const int2 src  = kk ? ec : en;
+            const int addr  = bp_base + (kk << 2);
+            J.off  = (unsigned)__builtin_amdgcn_ds_bpermute(addr, src.x) + jj16;
+            J.slot = __builtin_amdgcn_ds_bpermute(addr, src.y);
+            J.r    = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sr) + J.off);
+        };
+        JL A, B;
+        {
+            A.off  = (unsigned)__builtin_amdgcn_ds_bpermute(bp_base, ec.x) + jj16;
+            A.slot = __builtin_amdgcn_ds_bpermute(bp_base, ec.y);
+            A.r    = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sr) + A.off);
+        }
+        int k0 = 0;
+        auto advance = [&]() {
+            k0 += CPG;
+            if (!(k0 & 15)) {
+                ec = en;
+                en = entry(k0 + 16 + sub);
+            }
+        };
+        fetch(CPG, B);
+        use(A, true);
+        advance();
+        while (k0 < nmax) {
+            fetch(k0 + CPG, A);
+            use(B, false);
+            advance();
+            if (k0 >= nmax) break;
+            fetch(k0 + CPG, B);
+            use(A, false);
+            advance();
+        }
+        // the i-forces of the round: reduce over the 16 lanes of each group
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                fx[a] += __shfl_xor(fx[a], o, 64);
+                fy[a] += __shfl_xor(fy[a], o, 64);
+                fz[a] += __shfl_xor(fz[a], o, 64);
+            }
+        if (live && sub < 4) {
+            float rx = 0.f, ry = 0.f, rz = 0.f;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                if (a == sub) {
+                    rx = fx[a];
+                    ry = fy[a];
+                    rz = fz[a];
+                }
+            const int si      = ci * 4 + sub;
+            fown[3 * si]      = rx;
+            fown[3 * si + 1]  = ry;
+            fown[3 * si + 2]  = rz;
+        }
+    }
+    __syncthreads();
+    // the tile's records, written once (FP32)
+    float* out = rec + (size_t)rec_off[t] * RF;
+    for (int k = threadIdx.x; k < ntab * RF; k += blockDim.x)
+        out[k] = (float)((double)(long long)s_table[k] * (1.0 / FIX_ONE));
+    if (EV) {
+        double ed = e, wd = w;
+        for (int o = 32; o > 0; o >>= 1) {
+            ed += __shfl_xor(ed, o, 64);
+            wd += __shfl_xor(wd, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&ev[0], ed);
+            atomicAdd(&ev[1], wd);
+        }
+    }
+    (void)ucap;
+}
+
+// Each local atom's total force: its i-force plus the records of its j-cluster in record
+// order; then, when v is given, the final kick and, when xn is given, the next step's half
+// kick and drift (the same operations as the tail of k_cp_force_f32_full).
+template <int N>
+__global__ void k_cp_gather_half(int nslot_loc, int M, const int* __restrict__ ci_natoms,
+    const float* __restrict__ fown, const float* __restrict__ rec,
+    const int* __restrict__ rstart, const int* __restrict__ rev, float* __restrict__ f,
+    const double* __restrict__ sx, const int* __restrict__ st, double* __restrict__ v,
+    double dtforce, double* __restrict__ xn, float4* __restrict__ rn, double dt, RecGrid rg,
+    int* __restrict__ status)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot_loc || s % M >= ci_natoms[s / M]) return;
+    const int c = s / N, jj = s - c * N;
+    float rx = fown[3 * s], ry = fown[3 * s + 1], rz = fown[3 * s + 2];
+    const int e = rstart[c + 1];
+    for (int p = rstart[c]; p < e; p++) {
+        const float* q = rec + ((size_t)rev[p] * N + jj) * 3;
+        rx += q[0];
+        ry += q[1];
+        rz += q[2];
+    }
+    f[3 * s]     = rx;
+    f[3 * s + 1] = ry;
+    f[3 * s + 2] = rz;
+    if (v) {
+        double vx = v[3 * s] + dtforce * (double)rx;
+        double vy = v[3 * s + 1] + dtforce * (double)ry;
+        double vz = v[3 * s + 2] + dtforce * (double)rz;
+        if (xn) {
+            vx = vx + dtforce * (double)rx;
+            vy = vy + dtforce * (double)ry;
+            vz = vz + dtforce * (double)rz;
+            const double nxp = sx[3 * s] + dt * vx, nyp = sx[3 * s + 1] + dt * vy,
+                         nzp = sx[3 * s + 2] + dt * vz;
+            xn[3 * s]     = nxp;
+            xn[3 * s + 1] = nyp;
+            xn[3 * s + 2] = nzp;
+            rn[s]         = make_rec(nxp, nyp, nzp, st[s], rg, status);
+        }
+        v[3 * s]     = vx;
+        v[3 * s + 1] = vy;
+        v[3 * s + 2] = vz;
+    }
+}
+
 // ---------------------------------------------------------------- integrate (integrate.c:29-86)
 template <class FT>
 __global__ void k_cp_initial(int n3, int M, const int* __restrict__ ci_natoms, double* __restrict__ x,
@@ -1870,7 +2370,9 @@ void freeDeviceClusterGFX950(void)
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
         C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
-        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn, C.d_sr, C.d_srn };
+        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn, C.d_sr, C.d_srn,
+        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_rec_off, C.d_rcount, C.d_rstart, C.d_rcur,
+        C.d_rev, C.d_rec, C.d_fown };
     for (void* p : ptrs)
         dfree(p);
     if (C.h_status) (void)hipHostFree(C.h_status);
@@ -2206,6 +2708,7 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
 void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 {
     if (!ok()) return;
+    C.tiles_valid = false;
     TimedRegion tr(F_NEIGHBOR);
     hipStream_t s = stream();
     const int nci = C.nci;
@@ -2259,6 +2762,7 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
     (void)param;
     (void)atom;
     if (!ok()) return;
+    C.tiles_valid = false;
     TimedRegion tr(F_NEIGHBOR);
     hipStream_t s = stream();
     const int nci = C.nci;
@@ -2278,6 +2782,50 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
     if (C.h_status[0] & ST_NB_OVERFLOW)
         set_error(E_CAPACITY, "cluster-pair prune: a list longer than %d entries", PRUNE_CAP);
     neighbor->maxneighs = C.maxneighs;
+}
+
+// Tile tables and the reverse map of the current half list (see k_cp_tile_table).  A tile that
+// outgrows its tables leaves tiles_ok false: that list runs the atomic half-list kernel.
+constexpr int ST_TILE_WORD = 15;  // status word of the tile-table kernel
+static void build_tiles()
+{
+    hipStream_t s = stream();
+    const int nci = C.nci, ncj = C.ncj, N = C.N;
+    const int ntiles = (nci + TILE_CI - 1) / TILE_CI, ucap = TILE_W64 / (3 * N);
+    C.ntiles = ntiles;
+    C.tile_ucap = ucap;
+    C.tiles_valid = true;
+    C.tiles_ok    = false;
+    if (ntiles == 0) return;
+    const size_t nrec = (size_t)ntiles * ucap;
+    grow_n(&C.d_tile_cj, &C.cap_tile_cj, nrec, 0);
+    grow_n(&C.d_tile_n, &C.cap_tile_n, (size_t)ntiles + 1, 0);
+    grow_n(&C.d_eslot, &C.cap_eslot, (size_t)nci * C.maxneighs + 1, 0);
+    grow_n(&C.d_rec_off, &C.cap_rec_off, (size_t)ntiles + 1, 0);
+    grow_n(&C.d_rcount, &C.cap_rcount, (size_t)ncj + 1, 0);
+    grow_n(&C.d_rstart, &C.cap_rstart, (size_t)ncj + 1, 0);
+    grow_n(&C.d_rcur, &C.cap_rcur, (size_t)ncj + 1, 0);
+    grow_n(&C.d_rev, &C.cap_rev, nrec, 0);
+    grow_n(&C.d_rec, &C.cap_rec, nrec * 3 * N, 0);
+    grow_n(&C.d_fown, &C.cap_fown, 3 * (size_t)nci * C.M, 0);
+    if (failed()) return;
+    fill_i32(C.d_status + ST_TILE_WORD, 0, 1);
+    k_cp_tile_table<<<ntiles, 1024, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
+        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD);
+    MDG_LAUNCH_CHECK("cp_tile_table");
+    read_status();
+    if (failed() || C.h_status[ST_TILE_WORD]) return;
+    exclusive_scan(C.d_tile_n, C.d_rec_off, ntiles, C.d_rec_off + ntiles);
+    fill_i32(C.d_rcount, 0, (size_t)ncj + 1);
+    fill_i32(C.d_rcur, 0, (size_t)ncj + 1);
+    k_cp_rev_count<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap,
+        C.d_tile_cj, C.d_tile_n, C.d_rcount);
+    exclusive_scan(C.d_rcount, C.d_rstart, ncj, C.d_rstart + ncj);
+    k_cp_rev_fill<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap,
+        C.d_tile_cj, C.d_tile_n, C.d_rec_off, C.d_rstart, C.d_rcur, C.d_rev);
+    k_cp_rev_sort<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.d_rstart, C.d_rev);
+    MDG_LAUNCH_CHECK("cp_rev_map");
+    C.tiles_ok = !failed();
 }
 
 // The force call; with fuse_final the FP32 full-list kernel also applies finalIntegrate to each
@@ -2328,7 +2876,12 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     const int nci = C.nci, half = neighbor->half_neigh;
     const size_t nloc = (size_t)nci * C.M;
     if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
-    const bool recs = C.precision == 0 && !half;  // the FP32 full-list kernel reads records
+    // FP32 half lists (plain sums): tile tables of the current list, then the tiled kernels
+    if (C.precision == 0 && half && !C.comp && !C.tiles_valid) build_tiles();
+    if (failed()) return 0;
+    const bool tiled = C.precision == 0 && half && !C.comp && C.tiles_ok;
+    // the FP32 full-list and tiled half-list kernels read records
+    const bool recs = C.precision == 0 && (!half || tiled);
     if (recs) {
         // the records of every slot, rebuilt from the positions whenever something other than
         // the integrating force launch and the ghost update has moved them
@@ -2399,10 +2952,11 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     } else {
         grow_n(&C.d_sf32, &C.cap_sf32, 3 * nslots_all(), 0);
         if (failed()) return 0;
-        if (half) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
+        if (half && !tiled) fill_f32(C.d_sf32, 0.0f, 3 * nslots_all());
         unsigned grid = (blocks_for(nci, BLOCK / 64) + 7) & ~7u;  // multiple of 8, see kernel
-        // full lists: four i-clusters per wavefront (and the fused final integrate)
-        fused          = (fuse_final && !half) ? FUSED_FINAL | (xn ? FUSED_INITIAL : 0) : 0;
+        // full lists: four i-clusters per wavefront; tiled half lists: the gather pass (both with
+        // the fused final integrate)
+        fused          = (fuse_final && (!half || tiled)) ? FUSED_FINAL | (xn ? FUSED_INITIAL : 0) : 0;
         double* fuse_v = fused ? C.d_sv : nullptr;
         unsigned grid4 = (blocks_for((nci + 3) / 4, BLOCK / 64) + 7) & ~7u;
 #define MDG_F32(NN, H, E, U, CO)                                                                 \
@@ -2437,7 +2991,31 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
             else MDG_F32_U(NN, false, false);                                                    \
         }                                                                                        \
     } while (0)
-        if (C.N == 4) MDG_F32_N(4);
+        if (tiled) {
+            const size_t lds = (size_t)C.tile_ucap * 3 * C.N * sizeof(unsigned long long);
+            const unsigned gg = blocks_for((long long)nloc, BLOCK);
+#define MDG_HT(NN, E, U)                                                                         \
+    k_cp_force_half_tiled<NN, E, U><<<C.ntiles, TILE_WAVES * 64, lds, s>>>(nci, C.ncj, C.dummy_cj, \
+        C.ntypes, C.t32, C.d_sx, C.d_sr, C.d_nb, C.d_eslot, C.d_numneigh, C.d_ci_bbox, C.rg,      \
+        C.maxneighs, C.tile_ucap, C.d_tile_n, C.d_rec_off, C.d_rec, C.d_fown, C.d_ev)
+#define MDG_HT_N(NN)                                                                             \
+    do {                                                                                         \
+        if (C.ev_on) {                                                                           \
+            if (C.uniform) MDG_HT(NN, true, true);                                               \
+            else MDG_HT(NN, true, false);                                                        \
+        } else {                                                                                 \
+            if (C.uniform) MDG_HT(NN, false, true);                                              \
+            else MDG_HT(NN, false, false);                                                       \
+        }                                                                                        \
+        k_cp_gather_half<NN><<<gg, BLOCK, 0, s>>>((int)nloc, C.M, C.d_ci_natoms, C.d_fown,       \
+            C.d_rec, C.d_rstart, C.d_rev, C.d_sf32, C.d_sx, C.d_st, fuse_v, param->dtforce, xn,  \
+            rn, param->dt, C.rg, C.d_status + ST_REC_WORD);                                      \
+    } while (0)
+            if (C.N == 4) MDG_HT_N(4);
+            else MDG_HT_N(8);
+#undef MDG_HT_N
+#undef MDG_HT
+        } else if (C.N == 4) MDG_F32_N(4);
         else MDG_F32_N(8);
 #undef MDG_F32_N
 #undef MDG_F32_U

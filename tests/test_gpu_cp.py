@@ -230,15 +230,18 @@ def test_fused_force_final_equals_separate_calls():
 
 
 @pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
-def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape):
+@pytest.mark.parametrize("half", [0, 1])
+def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape, half):
     """The run loop integrates the next step inside the force call
-    (computeForceFinalInitialClusterGFX950, two position buffers) except where a step's full-step
-    velocities are read; a traced run, which reads them every step and so fuses nothing, gives
-    the same trajectory bit for bit, across rebuilds (every 7 steps) and prunes (every 3)."""
+    (computeForceFinalInitialClusterGFX950, two position buffers; for half lists in the gather
+    pass) except where a step's full-step velocities are read; a traced run, which reads them
+    every step and so fuses nothing, gives the same trajectory bit for bit, across rebuilds
+    (every 7 steps) and prunes (every 3).  Half lists too: their tile sums are fixed point, so
+    they do not depend on the order in which wavefronts add."""
     P = product()
     M, N = shape
     kw = dict(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=7, prune_every=3,
-              cluster_m=M, cluster_n=N)
+              cluster_m=M, cluster_n=N, half_neigh=half)
     out = []
     for trace in (False, True):
         sim = P.CPSim(**kw)
@@ -430,6 +433,44 @@ def test_full_size_against_oracle(shape):
     assert np.all(np.isfinite(errs[1] + errs[0]))
     assert max(errs[1]) <= 1e-12
     assert max(errs[0]) <= FP32_RMS_TOL
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
+def test_full_size_half_list_against_oracle(shape):
+    """64^3 cells (1,048,576 atoms) after 200 steps on a half list (the tiled pass: each cluster
+    pair once, j-forces summed per tile and gathered per atom): FP32-mixed forces against the
+    FP64 Verlet-list oracle on the same positions, 1e-6 relative RMS; the same positions on a full
+    list give the same forces to the FP32 bar."""
+    M, N = shape
+    P = product()
+    sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=1, ntimes=200)
+    try:
+        sim.run(200)
+        state = (sim.get("atoms_x"), sim.get("atoms_v"), sim.get("atoms_type"))
+    finally:
+        sim.close()
+    dev = {}
+    for half in (1, 0):
+        sim = P.CPSim(nx=64, ny=64, nz=64, cluster_m=M, cluster_n=N, precision=0, half_neigh=half,
+                      ntimes=0)
+        try:
+            sim.set_atoms(*state)
+            live = _live_slots(sim.get("ci_natoms"), M)
+            dev[half] = (sim.get("slot_x").reshape(-1, 3)[live],
+                         sim.get("slot_f").reshape(-1, 3)[live])
+        finally:
+            sim.close()
+    np.testing.assert_array_equal(dev[0][0], dev[1][0])
+    lat = (4 / 0.8442) ** (1 / 3)
+    xw = np.mod(dev[1][0], 64 * lat).reshape(-1)
+    one = np.ones(1)
+    fr, _, _ = _vl_oracle_forces(xw, np.zeros(xw.size // 3, np.int32), 64, lat, one, one,
+                                 one * 6.25, 1, cap_per_atom=120)
+    errs = [np.sqrt(((dev[h][1] - fr) ** 2).sum() / (fr ** 2).sum()) for h in (1, 0)]
+    print("1M atoms %dx%d FP32-mixed vs FP64 oracle: half list %.3e, full list %.3e"
+          % (M, N, errs[0], errs[1]), flush=True)
+    assert np.all(np.isfinite(errs))
+    assert max(errs) <= FP32_RMS_TOL
 
 
 KA = dict(ntypes=2, pair_epsilon=[1.0, 1.5, 1.5, 0.5], pair_sigma=[1.0, 0.8, 0.8, 0.88],
