@@ -1696,25 +1696,29 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
 // order in which the wavefronts of a tile add (and LDS integer atomics run at full rate, FP32
 // ones did not: 0.57 against 0.21 ms per 1M-atom call).  No global atomics anywhere.
 constexpr int TILE_CI    = 32;    // i-clusters per tile
-constexpr int TILE_ECAP  = 4096;  // list entries of one tile sorted in LDS
+constexpr int TILE_ECAP  = 2048;  // list entries per tile (more in any tile: the atomic kernel)
 constexpr int TILE_W64   = 7680;  // table capacity in 64-bit words (60 KB): 640 slots of 4x3,
                                   // 320 of 8x3
 constexpr int TILE_WAVES = 8;     // wavefronts per tile block, four i-clusters each per round
 constexpr double FIX_ONE = 1099511627776.0;  // 2^40, one force unit in table units
 static_assert(TILE_CI % (4 * TILE_WAVES) == 0, "whole rounds per tile");
 
-__global__ void __launch_bounds__(1024) k_cp_tile_table(int nci, int ncj, int ucap,
+constexpr int TILE_HASH = 2 * TILE_ECAP;  // open-addressing set of a tile's j-clusters (LDS)
+constexpr int TILE_TT   = 256;   // threads of the tile-table kernel
+__global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int ucap,
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
     int* __restrict__ tile_cj, int* __restrict__ tile_n, int* __restrict__ entry_slot,
     int* __restrict__ status)
 {
-    __shared__ int s_key[TILE_ECAP];
-    __shared__ int s_uni[TILE_ECAP];
     static_assert(TILE_CI <= 64, "the list-length prefix is one wavefront scan");
+    __shared__ int s_hash[TILE_HASH];
+    __shared__ int s_uni[TILE_HASH];
     __shared__ int s_pre[TILE_CI + 1];
     __shared__ int s_sh[33];
     const int t = blockIdx.x, c0 = t * TILE_CI, nc = min(nci - c0, TILE_CI);
     const int tid = threadIdx.x;
+    for (int k = tid; k < TILE_HASH; k += TILE_TT)
+        s_hash[k] = -1;
     // entries of the tile: prefix of the list lengths
     if (tid < 64) {
         int v = tid < nc ? numneigh[c0 + tid] : 0;
@@ -1732,45 +1736,34 @@ __global__ void __launch_bounds__(1024) k_cp_tile_table(int nci, int ncj, int uc
         if (tid == 0) atomicOr(status, 1);
         return;
     }
-    for (int e = tid; e < TILE_ECAP; e += blockDim.x) {
-        int key = INT_MAX;
-        if (e < E) {
-            int c = 0;
-            for (int step = TILE_CI / 2; step > 0; step >>= 1)
-                if (s_pre[c + step] <= e) c += step;
-            const int cj = nb[(size_t)(c0 + c) * maxneighs + (e - s_pre[c])];
-            key = cj < ncj ? cj : INT_MAX;
+    auto entry_at = [&](int e) {
+        int c = 0;
+        for (int step = TILE_CI / 2; step > 0; step >>= 1)
+            if (s_pre[c + step] <= e) c += step;
+        return (size_t)(c0 + c) * maxneighs + (e - s_pre[c]);
+    };
+    // the distinct local j-clusters into the set (linear probing; at most half full, as
+    // E <= TILE_ECAP = TILE_HASH / 2, so every probe sequence ends)
+    for (int e = tid; e < E; e += TILE_TT) {
+        const int cj = nb[entry_at(e)];
+        if (cj >= ncj) continue;
+        unsigned h = ((unsigned)cj * 2654435761u) & (TILE_HASH - 1);
+        for (;;) {
+            const int old = atomicCAS(&s_hash[h], -1, cj);
+            if (old == -1 || old == cj) break;
+            h = (h + 1) & (TILE_HASH - 1);
         }
-        s_key[e] = key;
     }
     __syncthreads();
-    // bitonic sort, ascending
-    for (int k = 2; k <= TILE_ECAP; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < TILE_ECAP; i += blockDim.x) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const int a = s_key[i], b = s_key[l];
-                    if (((i & k) == 0) == (a > b)) {
-                        s_key[i] = b;
-                        s_key[l] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    // distinct keys: flags, a block scan over runs of four per thread, compaction
-    int fl[4], sum = 0;
-    for (int q = 0; q < 4; q++) {
-        const int i = tid * 4 + q;
-        const int k = s_key[i];
-        fl[q]       = (k != INT_MAX && (i == 0 || s_key[i - 1] != k)) ? 1 : 0;
-        sum += fl[q];
-    }
+    // compaction: each thread a contiguous run of the set, block scan of the counts
+    constexpr int RUN = TILE_HASH / TILE_TT;
+    int cnt = 0;
+    for (int q = 0; q < RUN; q++)
+        cnt += s_hash[tid * RUN + q] >= 0;
     int tot;
     {
-        const int lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
-        int inc = sum;
+        const int lane = tid & 63, w = tid >> 6, nw = TILE_TT >> 6;
+        int inc = cnt;
         for (int o = 1; o < 64; o <<= 1) {
             int u = __shfl_up(inc, o, 64);
             if (lane >= o) inc += u;
@@ -1788,27 +1781,45 @@ __global__ void __launch_bounds__(1024) k_cp_tile_table(int nci, int ncj, int uc
             if (lane == nw - 1) s_sh[32] = vi;
         }
         __syncthreads();
-        int off = inc - sum + s_sh[w];
+        int off = inc - cnt + s_sh[w];
         tot     = s_sh[32];
-        for (int q = 0; q < 4; q++) {
-            if (fl[q]) s_uni[off] = s_key[tid * 4 + q];
-            off += fl[q];
+        for (int q = 0; q < RUN; q++) {
+            const int k = s_hash[tid * RUN + q];
+            if (k >= 0) s_uni[off++] = k;
         }
     }
-    __syncthreads();
     if (tot > ucap) {
         if (tid == 0) atomicOr(status, 2);
         return;
     }
-    for (int u = tid; u < tot; u += blockDim.x)
+    // sort the distinct keys (bitonic over the next power of two, padded with INT_MAX)
+    int P = 64;
+    while (P < tot)
+        P <<= 1;
+    __syncthreads();
+    for (int k = tot + tid; k < P; k += TILE_TT)
+        s_uni[k] = INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < P; i += TILE_TT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = s_uni[i], b = s_uni[l];
+                    if (((i & k) == 0) == (a > b)) {
+                        s_uni[i] = b;
+                        s_uni[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int u = tid; u < tot; u += TILE_TT)
         tile_cj[(size_t)t * ucap + u] = s_uni[u];
     if (tid == 0) tile_n[t] = tot;
     // every entry's slot: its j-cluster's place in the table (binary search)
-    for (int e = tid; e < E; e += blockDim.x) {
-        int c = 0;
-        for (int step = TILE_CI / 2; step > 0; step >>= 1)
-            if (s_pre[c + step] <= e) c += step;
-        const size_t at = (size_t)(c0 + c) * maxneighs + (e - s_pre[c]);
+    for (int e = tid; e < E; e += TILE_TT) {
+        const size_t at = entry_at(e);
         const int cj    = nb[at];
         int slot        = -1;
         if (cj < ncj) {
@@ -2810,7 +2821,7 @@ static void build_tiles()
     grow_n(&C.d_fown, &C.cap_fown, 3 * (size_t)nci * C.M, 0);
     if (failed()) return;
     fill_i32(C.d_status + ST_TILE_WORD, 0, 1);
-    k_cp_tile_table<<<ntiles, 1024, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
+    k_cp_tile_table<<<ntiles, TILE_TT, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
         C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD);
     MDG_LAUNCH_CHECK("cp_tile_table");
     read_status();
