@@ -802,12 +802,20 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 // One wavefront per i-cluster.  Each stencil column is sorted by bbminz, so the j-clusters
 // whose z-gap to the i-cluster alone exceeds the list cutoff form a prefix and a suffix of the
 // column; the column's coarse z index cuts most of them off (a j-cluster's bbmaxz is at most its
-// bbminz plus the largest cluster height).  Dropped candidates are exactly ones the bounding-box test would
-// reject, so the list is the reference's, in the reference's order (stencil columns in order,
-// clusters in column order).  The remaining candidates of up to 64 columns are flattened and
-// tested 64 at a time; ballots give every accepted j-cluster its place in the list.
+// bbminz plus the largest cluster height).  Dropped candidates are exactly ones the
+// bounding-box test would reject, so the list is the reference's, in the reference's order
+// (stencil columns in order, clusters in column order).  The remaining candidates of up to 64
+// columns are flattened and handled in segments of NB_SEG, in three passes:
+//   1. the bounding-box test of every candidate (its list entry and its box, stored in column
+//      order, load side by side), 64 at a time: reject, accept, or "decided by the atoms";
+//   2. the atom-pair test of the undecided ones only, queued in LDS and taken 64 at a time
+//      (all N positions of a candidate load at once);
+//   3. the accepted ones in candidate order, ballots giving each its place in the list.
+// The undecided third or so of the candidates then costs one atom-pair pass instead of one per
+// batch in which any lane needed it.
+constexpr int NB_SEG = 256;
 template <int M, int N>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(M == 4 && N == 4 ? 6 : 1))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
+__global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
@@ -816,8 +824,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(M ==
     int maxneighs, int* __restrict__ nb, int* __restrict__ numneigh, int* __restrict__ nmasked,
     int* __restrict__ status)
 {
+    enum : unsigned char { C_REJECT = 0, C_ACCEPT = 1, C_ATOMS = 2 };
     __shared__ int s_st[1024];
     __shared__ int s_mpos[BLOCK / 64][4];
+    __shared__ int s_cj[BLOCK / 64][NB_SEG];
+    __shared__ unsigned char s_fl[BLOCK / 64][NB_SEG];
+    __shared__ short s_q[BLOCK / 64][NB_SEG];
     for (int k = threadIdx.x; k < nstencil; k += blockDim.x)
         s_st[k] = stencil[k];
     __syncthreads();
@@ -870,71 +882,94 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(M ==
         }
         const int total = __shfl(inc, 63, 64);
         const int pre   = inc - cnt;
-        for (int t0 = 0; t0 < total; t0 += 64) {
-            const int t = t0 + lane;
-            // largest column c with pre[c] <= t: that column holds candidate t (columns with no
-            // candidates share their successor's pre and are skipped over)
-            int c = 0;
+        for (int g0 = 0; g0 < total; g0 += NB_SEG) {
+            const int segn = min(NB_SEG, total - g0);
+            int nq         = 0;
+            // 1. bounding-box tests
+            for (int b0 = 0; b0 < segn; b0 += 64) {
+                const int k = b0 + lane;
+                const int t = g0 + k;
+                // largest column c with pre[c] <= t: that column holds candidate t (columns with
+                // no candidates share their successor's pre and are skipped over)
+                int c = 0;
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int pc = __shfl(pre, c + step, 64);
-                if (pc <= t) c += step;
-            }
-            const int m = __shfl(lo, c, 64) + (t - __shfl(pre, c, 64));
-            bool pass = false, masked = false;
-            int cj    = -1;
-            if (t < total) {
-                // the box distance from the box in column order, loaded beside the entry (not
-                // after it) and computed before the half-list test needs the entry
-                const double* jbb = &jbox[6 * m];
-                double d2         = 0.0;
-#pragma unroll
-                for (int d = 2; d >= 0; d--) {
-                    double dl  = ib[2 * d] - jbb[2 * d + 1];
-                    double dh  = jbb[2 * d] - ib[2 * d + 1];
-                    double dm0 = fmax(fmax(dl, dh), 0.0);
-                    d2 += dm0 * dm0;
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int pc = __shfl(pre, c + step, 64);
+                    if (pc <= t) c += step;
                 }
-                cj = jlist[m];
-                if (!(half && ci_cj0 > cj)) {
-                    if (d2 < cutneighsq) {
-                        pass = d2 < rbb_sq;
-                        if (!pass) {
-                            // any atom pair inside the cutoff: all N positions load at once
-                            double xj[N][3];
+                const int m = __shfl(lo, c, 64) + (t - __shfl(pre, c, 64));
+                unsigned char fl = C_REJECT;
+                if (k < segn) {
+                    // the box in column order, loaded beside the entry rather than after it
+                    const double* jbb = &jbox[6 * m];
+                    double d2         = 0.0;
 #pragma unroll
-                            for (int cjj = 0; cjj < N; cjj++)
-#pragma unroll
-                                for (int d = 0; d < 3; d++)
-                                    xj[cjj][d] = sx[3 * (cj * N + cjj) + d];
-#pragma unroll
-                            for (int cii = 0; cii < M; cii++)
-#pragma unroll
-                                for (int cjj = 0; cjj < N; cjj++) {
-                                    double dx = xi[cii][0] - xj[cjj][0];
-                                    double dy = xi[cii][1] - xj[cjj][1];
-                                    double dz = xi[cii][2] - xj[cjj][2];
-                                    double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
-                                                          : dx * dx + dy * dy + dz * dz;
-                                    pass = pass || rsq < cutneighsq;
-                                }
-                        }
+                    for (int d = 2; d >= 0; d--) {
+                        double dl  = ib[2 * d] - jbb[2 * d + 1];
+                        double dh  = jbb[2 * d] - ib[2 * d + 1];
+                        double dm0 = fmax(fmax(dl, dh), 0.0);
+                        d2 += dm0 * dm0;
                     }
-                    masked = pass && imask_of(rule, ci, cj) != IMASK_ALL;
+                    const int cj = jlist[m];
+                    if (!(half && ci_cj0 > cj) && d2 < cutneighsq)
+                        fl = d2 < rbb_sq ? C_ACCEPT : C_ATOMS;
+                    s_cj[w][k] = cj;
+                    s_fl[w][k] = fl;
+                }
+                const unsigned long long qb = __ballot(fl == C_ATOMS);
+                if (fl == C_ATOMS) s_q[w][nq + __popcll(qb & lt)] = (short)k;
+                nq += __popcll(qb);
+            }
+            // one wavefront: its LDS operations complete in order
+            __builtin_amdgcn_wave_barrier();
+            // 2. atom-pair tests of the undecided candidates
+            for (int q0 = 0; q0 < nq; q0 += 64) {
+                if (q0 + lane < nq) {
+                    const int k  = s_q[w][q0 + lane];
+                    const int cj = s_cj[w][k];
+                    // any atom pair inside the cutoff: all N positions load at once
+                    double xj[N][3];
+#pragma unroll
+                    for (int cjj = 0; cjj < N; cjj++)
+#pragma unroll
+                        for (int d = 0; d < 3; d++)
+                            xj[cjj][d] = sx[3 * (cj * N + cjj) + d];
+                    bool pass = false;
+#pragma unroll
+                    for (int cii = 0; cii < M; cii++)
+#pragma unroll
+                        for (int cjj = 0; cjj < N; cjj++) {
+                            double dx = xi[cii][0] - xj[cjj][0];
+                            double dy = xi[cii][1] - xj[cjj][1];
+                            double dz = xi[cii][2] - xj[cjj][2];
+                            double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                                  : dx * dx + dy * dy + dz * dz;
+                            pass = pass || rsq < cutneighsq;
+                        }
+                    s_fl[w][k] = pass ? C_ACCEPT : C_REJECT;
                 }
             }
-            const unsigned long long bal  = __ballot(pass);
-            const unsigned long long mbal = __ballot(masked);
-            if (pass) {
-                int pos = n + __popcll(bal & lt);
-                if (pos < maxneighs) out[pos] = cj;
-                if (masked) {
-                    int k = nm + __popcll(mbal & lt);
-                    if (k < 4) s_mpos[w][k] = pos;
+            __builtin_amdgcn_wave_barrier();
+            // 3. the accepted candidates in order
+            for (int b0 = 0; b0 < segn; b0 += 64) {
+                const int k       = b0 + lane;
+                const bool pass   = k < segn && s_fl[w][k] == C_ACCEPT;
+                const int cj      = pass ? s_cj[w][k] : -1;
+                const bool masked = pass && imask_of(rule, ci, cj) != IMASK_ALL;
+                const unsigned long long bal  = __ballot(pass);
+                const unsigned long long mbal = __ballot(masked);
+                if (pass) {
+                    int pos = n + __popcll(bal & lt);
+                    if (pos < maxneighs) out[pos] = cj;
+                    if (masked) {
+                        int kk = nm + __popcll(mbal & lt);
+                        if (kk < 4) s_mpos[w][kk] = pos;
+                    }
                 }
+                n += __popcll(bal);
+                nm += __popcll(mbal);
             }
-            n += __popcll(bal);
-            nm += __popcll(mbal);
+            __builtin_amdgcn_wave_barrier();
         }
     }
     // single-lane epilogue: masked entries move to the front, in order of arrival
