@@ -155,8 +155,8 @@ struct CPState {
     double dt = 0, dtforce = 0, mass = 1.0;
     bool forces_are_f32 = false;
     // reductions
-    double *d_red = nullptr, *d_ev = nullptr;
-    size_t cap_red = 0;
+    double *d_red = nullptr, *d_ev = nullptr, *d_evp = nullptr;  // d_evp: per-wavefront pairs
+    size_t cap_red = 0, cap_evp = 0;
     int* d_status = nullptr;
     int* h_status = nullptr;
     double* h_red = nullptr;
@@ -1068,6 +1068,52 @@ __global__ void __launch_bounds__(BLOCK) k_cp_prune(int nci, int fma_dist,
         lst[k] = s_e[w][k];
 }
 
+// ---------------------------------------------------------------- energy and virial sums
+// Every wavefront of a force launch leaves its energy and virial sums in its own pair of slots
+// (cleared before the launch, so a wavefront that has no i-cluster leaves zeros); k_cp_sum_ev
+// adds the slots in a fixed order.  No atomics: one FP64 atomic pair per wavefront on one
+// address cost 1.4 ms per 1M-atom call, and the sum now does not depend on timing.
+__device__ __forceinline__ void ev_store(double e, double w, double* __restrict__ part)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        e += __shfl_xor(e, o, 64);
+        w += __shfl_xor(w, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const size_t k = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        part[2 * k]     = e;
+        part[2 * k + 1] = w;
+    }
+}
+
+// ev[0..1] = the sums of the nw slot pairs: per thread a strided sum, then a fixed tree
+constexpr int EVR_T = 1024;
+__global__ void __launch_bounds__(EVR_T) k_cp_sum_ev(size_t nw, const double* __restrict__ part,
+    double* __restrict__ ev)
+{
+    __shared__ double s[2][EVR_T];
+    const int t = threadIdx.x;
+    double e = 0.0, w = 0.0;
+    for (size_t k = t; k < nw; k += EVR_T) {
+        e += part[2 * k];
+        w += part[2 * k + 1];
+    }
+    s[0][t] = e;
+    s[1][t] = w;
+    __syncthreads();
+    for (int o = EVR_T / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            s[0][t] += s[0][t + o];
+            s[1][t] += s[1][t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        ev[0] = s[0][0];
+        ev[1] = s[1][0];
+    }
+}
+
 // ---------------------------------------------------------------- FP64 force, reference arithmetic
 template <int M, int N, bool HALF, bool EV>
 __global__ void __launch_bounds__(BLOCK) k_cp_force_f64(int nci, int ncj, int ntypes, int uniform,
@@ -1075,7 +1121,6 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f64(int nci, int ncj, int nt
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
     double* __restrict__ f, double* __restrict__ ev)
 {
-    __shared__ double s_red[2][BLOCK / 64];
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
     const int ci = si / M;
     double e = 0.0, w = 0.0;
@@ -1131,16 +1176,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f64(int nci, int ncj, int nt
             f[3 * si + 2] = fz;
         }
     }
-    if (EV) {
-        for (int o = 32; o > 0; o >>= 1) {
-            e += __shfl_xor(e, o, 64);
-            w += __shfl_xor(w, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&ev[0], e);
-            atomicAdd(&ev[1], w);
-        }
-    }
+    if (EV) ev_store(e, w, ev);
 }
 
 // ---------------------------------------------------------------- FP32-mixed force kernel
@@ -1374,17 +1410,7 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
             f[3 * si + 2] = rz;
         }
     }
-    if (EV) {
-        double ed = e, wd = w;
-        for (int o = 32; o > 0; o >>= 1) {
-            ed += __shfl_xor(ed, o, 64);
-            wd += __shfl_xor(wd, o, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&ev[0], ed);
-            atomicAdd(&ev[1], wd);
-        }
-    }
+    if (EV) ev_store(e, w, ev);
 }
 
 // Full lists, j-atom lanes.  One wavefront works on four i-clusters: the 16 lanes of group g
@@ -1705,17 +1731,7 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             v[3 * si + 2] = vz;
         }
     }
-    if (EV) {
-        double ed = e, wd = w;
-        for (int o = 32; o > 0; o >>= 1) {
-            ed += __shfl_xor(ed, o, 64);
-            wd += __shfl_xor(wd, o, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&ev[0], ed);
-            atomicAdd(&ev[1], wd);
-        }
-    }
+    if (EV) ev_store(e, w, ev);
     (void)ncj;
 }
 
@@ -2159,17 +2175,7 @@ __global__ void __launch_bounds__(TILE_WAVES * 64) k_cp_force_half_tiled(int nci
     float* out = rec + (size_t)rec_off[t] * RF;
     for (int k = threadIdx.x; k < ntab * RF; k += blockDim.x)
         out[k] = (float)((double)(long long)s_table[k] * (1.0 / FIX_ONE));
-    if (EV) {
-        double ed = e, wd = w;
-        for (int o = 32; o > 0; o >>= 1) {
-            ed += __shfl_xor(ed, o, 64);
-            wd += __shfl_xor(wd, o, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&ev[0], ed);
-            atomicAdd(&ev[1], wd);
-        }
-    }
+    if (EV) ev_store(e, w, ev);
     (void)ucap;
 }
 
@@ -2416,7 +2422,7 @@ void freeDeviceClusterGFX950(void)
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
         C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
         C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
-        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_status, C.d_sxn, C.d_sr, C.d_srn,
+        C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_evp, C.d_status, C.d_sxn, C.d_sr, C.d_srn,
         C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_rec_off, C.d_rcount, C.d_rstart, C.d_rcur,
         C.d_rev, C.d_rec, C.d_fown };
     for (void* p : ptrs)
@@ -2921,7 +2927,6 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     hipStream_t s   = stream();
     const int nci = C.nci, half = neighbor->half_neigh;
     const size_t nloc = (size_t)nci * C.M;
-    if (C.ev_on) fill_f64(C.d_ev, 0.0, 2);
     // FP32 half lists (plain sums): tile tables of the current list, then the tiled kernels
     if (C.precision == 0 && half && !C.comp && !C.tiles_valid) build_tiles();
     if (failed()) return 0;
@@ -2970,15 +2975,27 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
             nslots_all());
         return 0;
     }
+    // energy and virial: one slot pair per wavefront of the force launch, cleared before it
+    size_t ev_nw = 0;
+    auto ev_parts = [&](size_t nw) -> double* {
+        if (!C.ev_on) return C.d_ev;
+        grow_n(&C.d_evp, &C.cap_evp, 2 * nw, 0);
+        if (failed()) return nullptr;
+        fill_f64(C.d_evp, 0.0, 2 * nw);
+        ev_nw = nw;
+        return C.d_evp;
+    };
     if (C.precision == 1) {
         grow_n(&C.d_sf64, &C.cap_sf64, 3 * nslots_all(), 0);
         if (failed()) return 0;
         if (half) fill_f64(C.d_sf64, 0.0, 3 * nslots_all());
         unsigned grid = blocks_for((long long)nloc, BLOCK);
+        double* evp   = ev_parts((size_t)grid * (BLOCK / 64));
+        if (failed()) return 0;
         const int uni = C.uniform ? 1 : 0;
 #define MDG_F64(MM, NN, H, E)                                                                    \
     k_cp_force_f64<MM, NN, H, E><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.ntypes, uni, C.t64, C.d_sx, \
-        C.d_st, C.d_nb, C.d_numneigh, C.maxneighs, C.d_sf64, C.d_ev)
+        C.d_st, C.d_nb, C.d_numneigh, C.maxneighs, C.d_sf64, evp)
 #define MDG_F64_SHAPE(MM, NN)                                                                    \
     do {                                                                                         \
         if (half) {                                                                              \
@@ -3005,16 +3022,19 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         fused          = (fuse_final && (!half || tiled)) ? FUSED_FINAL | (xn ? FUSED_INITIAL : 0) : 0;
         double* fuse_v = fused ? C.d_sv : nullptr;
         unsigned grid4 = (blocks_for((nci + 3) / 4, BLOCK / 64) + 7) & ~7u;
+        double* evp    = ev_parts(tiled ? (size_t)C.ntiles * TILE_WAVES
+                                        : (size_t)(half ? grid : grid4) * (BLOCK / 64));
+        if (failed()) return 0;
 #define MDG_F32(NN, H, E, U, CO)                                                                 \
     do {                                                                                         \
         if (H)                                                                                   \
             k_cp_force_f32<NN, true, E, U, CO><<<grid, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
                 C.ntypes, C.t32, C.d_sx, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,             \
-                C.maxneighs, C.d_sf32, C.d_ev);                                                   \
+                C.maxneighs, C.d_sf32, evp);                                                      \
         else                                                                                     \
             k_cp_force_f32_full<NN, E, U, CO><<<grid4, BLOCK, 0, s>>>(nci, C.ncj, C.dummy_cj,     \
                 C.ntypes, C.t32, C.d_sx, C.d_sr, C.d_st, C.d_nb, C.d_numneigh, C.d_ci_natoms,     \
-                C.d_ci_bbox, C.rg, C.maxneighs, C.d_sf32, C.d_ev, fuse_v, param->dtforce, xn,    \
+                C.d_ci_bbox, C.rg, C.maxneighs, C.d_sf32, evp, fuse_v, param->dtforce, xn,       \
                 rn, param->dt, C.d_status + ST_REC_WORD);                                        \
     } while (0)
 #define MDG_F32_U(NN, H, E)                                                                      \
@@ -3043,7 +3063,7 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
 #define MDG_HT(NN, E, U)                                                                         \
     k_cp_force_half_tiled<NN, E, U><<<C.ntiles, TILE_WAVES * 64, lds, s>>>(nci, C.ncj, C.dummy_cj, \
         C.ntypes, C.t32, C.d_sx, C.d_sr, C.d_nb, C.d_eslot, C.d_numneigh, C.d_ci_bbox, C.rg,      \
-        C.maxneighs, C.tile_ucap, C.d_tile_n, C.d_rec_off, C.d_rec, C.d_fown, C.d_ev)
+        C.maxneighs, C.tile_ucap, C.d_tile_n, C.d_rec_off, C.d_rec, C.d_fown, evp)
 #define MDG_HT_N(NN)                                                                             \
     do {                                                                                         \
         if (C.ev_on) {                                                                           \
@@ -3069,6 +3089,10 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         C.forces_are_f32 = true;
     }
     MDG_LAUNCH_CHECK("cp_force");
+    if (ev_nw) {
+        k_cp_sum_ev<<<1, EVR_T, 0, s>>>(ev_nw, C.d_evp, C.d_ev);
+        MDG_LAUNCH_CHECK("cp_sum_ev");
+    }
     if (fused & FUSED_INITIAL) {
         std::swap(C.d_sx, C.d_sxn);
         std::swap(C.cap_sx, C.cap_sxn);
