@@ -116,6 +116,7 @@ struct CPState {
     // current list on the first FP32 force call after a build or prune (tiles_valid); tiles_ok
     // false when a tile outgrew its tables (that list then runs the atomic kernel)
     bool tiles_valid = false, tiles_ok = false;
+    bool tiles_wait = false;  // tile tables launched, their status not yet read
     int ntiles = 0, tile_ucap = 0;
     int *d_tile_cj = nullptr, *d_tile_n = nullptr, *d_eslot = nullptr, *d_rec_off = nullptr,
         *d_rcount = nullptr, *d_rstart = nullptr, *d_rcur = nullptr, *d_rev = nullptr;
@@ -381,6 +382,8 @@ __global__ void __launch_bounds__(SORT_T) k_cp_bin_sort(const int* __restrict__ 
     for (int p = t; p < c; p += SORT_T) {
         const double zp = s_z[p];
         int r = 0, e = 0, q = 0;
+        // four independent LDS reads in flight per round (the rolled loop waited on each)
+#pragma unroll 4
         for (; q + 2 <= c; q += 2) {
             const double2 zq = *reinterpret_cast<const double2*>(&s_z[q]);
             r += (zq.x < zp) + (zq.y < zp);
@@ -813,9 +816,12 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 //   3. the accepted ones in candidate order, ballots giving each its place in the list.
 // The undecided third or so of the candidates then costs one atom-pair pass instead of one per
 // batch in which any lane needed it.
+// Six wavefronts per SIMD (80 VGPRs, a few spilled) ran the 1M-atom build in 442-446 us against
+// 465-469 us at the five the compiler picks by itself (91 VGPRs); at seven (72 VGPRs, more
+// spilled) it took 557 us.
 constexpr int NB_SEG = 256;
 template <int M, int N>
-__global__ void __launch_bounds__(BLOCK) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
@@ -1770,9 +1776,10 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
     const int tid = threadIdx.x;
     for (int k = tid; k < TILE_HASH; k += TILE_TT)
         s_hash[k] = -1;
-    // entries of the tile: prefix of the list lengths
+    // entries of the tile: prefix of the list lengths (clamped: the kernel may run on a list
+    // whose build overflowed, before the host has seen that and rebuilt it)
     if (tid < 64) {
-        int v = tid < nc ? numneigh[c0 + tid] : 0;
+        int v = tid < nc ? min(numneigh[c0 + tid], maxneighs) : 0;
         int inc = v;
         for (int o = 1; o < 64; o <<= 1) {
             int u = __shfl_up(inc, o, 64);
@@ -2757,6 +2764,10 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
     MDG_LAUNCH_CHECK("cp_bin_j");
 }
 
+static void tiles_launch();
+static void tiles_finish();
+static void ensure_records();
+
 void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 {
     if (!ok()) return;
@@ -2792,6 +2803,12 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
         else MDG_NB(1, 2);
 #undef MDG_NB
         MDG_LAUNCH_CHECK("cp_build_nb");
+        // the tile tables of an FP32 half list and the records of the new slots are launched
+        // before the status read, which then serves them too (the force call would otherwise
+        // launch them after it, each launch then waiting on the host)
+        const bool half = neighbor->half_neigh != 0;
+        if (C.precision == 0 && half && !C.comp) tiles_launch();
+        if (C.precision == 0 && !(half && C.comp)) ensure_records();
         read_status();
         if (failed()) return;
         if (C.h_status[0] & (ST_BIN_RANGE | ST_JBIN_CAP)) {
@@ -2800,9 +2817,12 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
         }
         if (!(C.h_status[0] & ST_NB_OVERFLOW)) {
             neighbor->maxneighs = C.maxneighs;
+            tiles_finish();
             (void)atom;
             return;
         }
+        C.tiles_valid = false;
+        C.tiles_wait  = false;
         // neighbor.c:705-720: grow to 1.2x the largest count and rebuild
         C.maxneighs = (int)(C.h_status[1] * 1.2) + 1;
     }
@@ -2829,25 +2849,30 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
     else MDG_PRUNE(1, 2);
 #undef MDG_PRUNE
     MDG_LAUNCH_CHECK("cp_prune");
+    if (C.precision == 0 && neighbor->half_neigh && !C.comp) tiles_launch();  // as in the build
     read_status();
     if (failed()) return;
     if (C.h_status[0] & ST_NB_OVERFLOW)
         set_error(E_CAPACITY, "cluster-pair prune: a list longer than %d entries", PRUNE_CAP);
     neighbor->maxneighs = C.maxneighs;
+    tiles_finish();
 }
 
 // Tile tables and the reverse map of the current half list (see k_cp_tile_table).  A tile that
 // outgrows its tables leaves tiles_ok false: that list runs the atomic half-list kernel.
 constexpr int ST_TILE_WORD = 15;  // status word of the tile-table kernel
-static void build_tiles()
+// The tile-table kernel of the current list, before a status read: the list build and the
+// prune launch it before their own status read, so that one read serves both.
+static void tiles_launch()
 {
     hipStream_t s = stream();
     const int nci = C.nci, ncj = C.ncj, N = C.N;
     const int ntiles = (nci + TILE_CI - 1) / TILE_CI, ucap = TILE_W64 / (3 * N);
-    C.ntiles = ntiles;
-    C.tile_ucap = ucap;
+    C.ntiles      = ntiles;
+    C.tile_ucap   = ucap;
     C.tiles_valid = true;
     C.tiles_ok    = false;
+    C.tiles_wait  = false;
     if (ntiles == 0) return;
     const size_t nrec = (size_t)ntiles * ucap;
     grow_n(&C.d_tile_cj, &C.cap_tile_cj, nrec, 0);
@@ -2865,8 +2890,19 @@ static void build_tiles()
     k_cp_tile_table<<<ntiles, TILE_TT, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
         C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD);
     MDG_LAUNCH_CHECK("cp_tile_table");
-    read_status();
+    C.tiles_wait = true;
+}
+
+// After the status read that followed tiles_launch: the reverse map, unless a tile outgrew its
+// tables.
+static void tiles_finish()
+{
+    if (!C.tiles_wait) return;
+    C.tiles_wait = false;
     if (failed() || C.h_status[ST_TILE_WORD]) return;
+    hipStream_t s = stream();
+    const int ntiles = C.ntiles, ucap = C.tile_ucap, ncj = C.ncj;
+    const size_t nrec = (size_t)ntiles * ucap;
     exclusive_scan(C.d_tile_n, C.d_rec_off, ntiles, C.d_rec_off + ntiles);
     fill_i32(C.d_rcount, 0, (size_t)ncj + 1);
     fill_i32(C.d_rcur, 0, (size_t)ncj + 1);
@@ -2878,6 +2914,31 @@ static void build_tiles()
     k_cp_rev_sort<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.d_rstart, C.d_rev);
     MDG_LAUNCH_CHECK("cp_rev_map");
     C.tiles_ok = !failed();
+}
+
+static void build_tiles()
+{
+    tiles_launch();
+    if (!C.tiles_wait) return;
+    read_status();
+    tiles_finish();
+}
+
+// The records of every slot (the FP32 full-list and tiled half-list kernels read them), rebuilt
+// from the positions whenever something other than the integrating force launch and the ghost
+// update has moved them.
+static void ensure_records()
+{
+    const size_t nsl = nslots_all(), cap0 = C.cap_sr;
+    grow_n(&C.d_sr, &C.cap_sr, nsl, 0);
+    if (failed()) return;
+    if (C.cap_sr != cap0) C.rec_valid = false;
+    if (!C.rec_valid) {
+        k_cp_records<<<blocks_for((long long)nsl, BLOCK), BLOCK, 0, stream()>>>((int)nsl, C.d_sx,
+            C.d_st, C.rg, C.d_sr, C.d_status + ST_REC_WORD);
+        MDG_LAUNCH_CHECK("cp_records");
+        C.rec_valid = true;
+    }
 }
 
 // The force call; with fuse_final the FP32 full-list kernel also applies finalIntegrate to each
@@ -2933,20 +2994,8 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     const bool tiled = C.precision == 0 && half && !C.comp && C.tiles_ok;
     // the FP32 full-list and tiled half-list kernels read records
     const bool recs = C.precision == 0 && (!half || tiled);
-    if (recs) {
-        // the records of every slot, rebuilt from the positions whenever something other than
-        // the integrating force launch and the ghost update has moved them
-        const size_t nsl = nslots_all(), cap0 = C.cap_sr;
-        grow_n(&C.d_sr, &C.cap_sr, nsl, 0);
-        if (failed()) return 0;
-        if (C.cap_sr != cap0) C.rec_valid = false;
-        if (!C.rec_valid) {
-            k_cp_records<<<blocks_for((long long)nsl, BLOCK), BLOCK, 0, s>>>((int)nsl, C.d_sx,
-                C.d_st, C.rg, C.d_sr, C.d_status + ST_REC_WORD);
-            MDG_LAUNCH_CHECK("cp_records");
-            C.rec_valid = true;
-        }
-    }
+    if (recs) ensure_records();
+    if (failed()) return 0;
     // the next step's positions and their records go to the second buffers, whose padding,
     // dummy and ghost slots must match the current ones (copied once after each rebuild or
     // upload)
