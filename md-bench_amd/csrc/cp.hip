@@ -97,6 +97,9 @@ struct CPState {
     size_t cap_abin_of = 0, cap_abin_cnt = 0, cap_abin_start = 0, cap_abin_atoms = 0;
     // clusters
     int nci = 0, ncj = 0, nghost = 0, dummy_cj = 0, nghost_atoms = 0;
+    // after a binning, until the ghost setup (or any other entry point) reads the cluster count
+    // back: nci and ncj hold upper bounds, the count itself is status word 8 on the device
+    bool nci_pending = false;
     int *d_ncl = nullptr, *d_cstart = nullptr;
     size_t cap_ncl = 0, cap_cstart = 0;
     int *d_ci_natoms = nullptr, *d_ci_bin = nullptr, *d_cj_natoms = nullptr;
@@ -471,7 +474,7 @@ __global__ void k_cp_fill_ci_bin(int mbins, const int* __restrict__ cstart, cons
 // buildClustersCPU (neighbor.c:1513-1611): cluster ci holds atoms cl*M .. cl*M+M-1 of its bin.
 // One lane per slot; a cluster's M lanes are adjacent (M divides 64) and reduce its bounding box
 // and atom count with cross-lane reads.
-__global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bin,
+__global__ void k_cp_fill_clusters(const int* __restrict__ d_nci, int M, const int* __restrict__ ci_bin,
     const int* __restrict__ cstart, const int* __restrict__ abin_start,
     const int* __restrict__ abin_atoms, const double* __restrict__ ax,
     const double* __restrict__ av, const int* __restrict__ at, double* __restrict__ sx,
@@ -481,7 +484,7 @@ __global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bi
     const int s    = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int ci = s / M, cii = s % M;
-    const bool live = ci < nci;
+    const bool live = ci < *d_nci;  // the grid covers an upper bound of the count
     bool has        = false;
     double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
     if (live) {
@@ -525,12 +528,14 @@ __global__ void k_cp_fill_clusters(int nci, int M, const int* __restrict__ ci_bi
     }
 }
 
-// defineJClusters (neighbor.c:1807-1927), shapes M == N and N == 2M.
-__global__ void k_cp_define_j(int ncj, int M, int N, const int* __restrict__ ci_natoms,
-    const double* __restrict__ ci_bbox, int* __restrict__ cj_natoms, double* __restrict__ cj_bbox)
+// defineJClusters (neighbor.c:1807-1927), shapes M == N and N == 2M.  The j-cluster count is
+// nci * M / N with nci read from the device (the grid covers an upper bound).
+__global__ void k_cp_define_j(const int* __restrict__ d_nci, int M, int N,
+    const int* __restrict__ ci_natoms, const double* __restrict__ ci_bbox,
+    int* __restrict__ cj_natoms, double* __restrict__ cj_bbox)
 {
     int cj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cj >= ncj) return;
+    if (cj >= *d_nci * M / N) return;
     if (M == N) {
         cj_natoms[cj] = ci_natoms[cj];
         for (int k = 0; k < 6; k++)
@@ -547,15 +552,20 @@ __global__ void k_cp_define_j(int ncj, int M, int N, const int* __restrict__ ci_
 
 // ---------------------------------------------------------------- setupPbc (pbc.c:188-331)
 // ghost clusters per j-cluster, and the ghost atoms they hold summed into *natoms_ghost: one
-// atomic per block of GC_T threads (same-address atomics serialise at the memory side)
+// atomic per block of GC_T threads (same-address atomics serialise at the memory side).  The
+// grid covers ncj_cap counters; those past the j-cluster count (nci * M / N, nci read from the
+// device) are set to 0, so that a scan over all ncj_cap gives the same offsets and total.
 constexpr int GC_T = 1024;
-__global__ void __launch_bounds__(GC_T) k_cp_ghost_count(int ncj, const int* __restrict__ cj_natoms,
-    const double* __restrict__ cj_bbox, double xprd, double yprd, double zprd, double cut,
-    int* __restrict__ cnt, int* __restrict__ natoms_ghost)
+__global__ void __launch_bounds__(GC_T) k_cp_ghost_count(int ncj_cap, const int* __restrict__ d_nci,
+    int M, int N, const int* __restrict__ cj_natoms, const double* __restrict__ cj_bbox,
+    double xprd, double yprd, double zprd, double cut, int* __restrict__ cnt,
+    int* __restrict__ natoms_ghost)
 {
     __shared__ int s_n[GC_T / 64];
-    const int cj = blockIdx.x * blockDim.x + threadIdx.x;
-    int nadd     = 0;
+    const int cj  = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncj = *d_nci * M / N;
+    int nadd      = 0;
+    if (cj >= ncj && cj < ncj_cap) cnt[cj] = 0;
     if (cj < ncj) {
         const int na = cj_natoms[cj];
         const int c  = na > 0 ? __popc(ghost_mask_bb(&cj_bbox[6 * cj], xprd, yprd, zprd, cut)) : 0;
@@ -2298,7 +2308,16 @@ __global__ void __launch_bounds__(BLOCK) k_cp_sum_partials(int n, const double* 
 }
 
 // ---------------------------------------------------------------- host helpers
-bool ok() { return !failed() && C.ready; }
+void settle_counts();
+// Entry points that need the host-side cluster counts call ok(), which reads a pending count
+// back first; the rebuild steps between the binning and the ghost setup call queue_ok().
+bool queue_ok() { return !failed() && C.ready; }
+bool ok()
+{
+    if (!queue_ok()) return false;
+    if (C.nci_pending) settle_counts();
+    return !failed();
+}
 
 // The address of `scratch` on the current device, resolved on the first call per device.
 double* scratch_ptr()
@@ -2590,6 +2609,38 @@ void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy)
     MDG_LAUNCH_CHECK("cp_wrap");
 }
 
+namespace {
+// The status words written since the binning: binning flags (word 0), the cluster count (word
+// 8), the record range flag.  Called after the status read that follows the binning.
+void apply_cluster_counts()
+{
+    if (!C.nci_pending) return;
+    C.nci_pending = false;
+    if (C.h_status[0] & (ST_BIN_RANGE | ST_BIN_CAP)) {
+        set_error(E_BIN, "cluster binning failed (flags %d): an atom outside the grid or a "
+                         "column above %d atoms", C.h_status[0], BIN_CAP);
+        return;
+    }
+    if (C.h_status[ST_REC_WORD] & ST_REC_RANGE) {
+        set_error(E_BIN, "FP32 j-records: a position left the record grid (box +- %g)",
+            C.rg.bias * C.rg.h);
+        return;
+    }
+    C.nci = C.h_status[8];
+    C.ncj = C.nci * C.M / C.N;
+}
+
+void settle_counts()
+{
+    read_status();
+    if (!failed()) apply_cluster_counts();
+}
+}  // namespace
+
+// The cluster count is left on the device (status word 8): the j-cluster and ghost-count
+// kernels queue behind the binning without a host wait, sized for an upper bound of the count
+// (each bin holds ceil(atoms / M) clusters, one more when N = 2M rounds the count up to even),
+// and the ghost setup reads the count with the ghost totals.
 void buildClustersGFX950(Atom* atom)
 {
     if (!ok()) return;
@@ -2619,71 +2670,73 @@ void buildClustersGFX950(Atom* atom)
         C.d_abin_cnt, C.d_abin_atoms);
     k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
         C.d_status);
-    // the cluster count also to status word 8, read with the flags in one copy
+    // the cluster count to status word 8
     exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb, C.d_status + 8);
     MDG_LAUNCH_CHECK("cp_binning");
-    read_status();
-    if (failed()) return;
-    if (C.h_status[0] & (ST_BIN_RANGE | ST_BIN_CAP)) {
-        set_error(E_BIN, "cluster binning failed (flags %d): an atom outside the grid or a "
-                         "column above %d atoms", C.h_status[0], BIN_CAP);
+    const long long cap = ((long long)na + C.M - 1) / C.M + (long long)mb * (C.N > C.M ? 2 : 1);
+    if (cap * C.M + C.N >= (1ll << 31)) {
+        set_error(E_ARG, "cluster binning: %lld cluster slots exceed 32-bit indexing", cap * C.M);
         return;
     }
-    if (C.h_status[ST_REC_WORD] & ST_REC_RANGE) {
-        set_error(E_BIN, "FP32 j-records: a position left the record grid (box +- %g)",
-            C.rg.bias * C.rg.h);
-        return;
-    }
-    const int nci = C.h_status[8];
-    C.nci         = nci;
-    C.ncj = nci * C.M / C.N;
-    grow_n(&C.d_ci_natoms, &C.cap_ci_natoms, (size_t)nci + 1, 0);
-    grow_n(&C.d_ci_bin, &C.cap_ci_bin, (size_t)nci + 1, 0);
-    grow_n(&C.d_ci_bbox, &C.cap_ci_bbox, 6 * (size_t)nci + 6, 0);
-    grow_n(&C.d_sv, &C.cap_sv, 3 * (size_t)nci * C.M + 3, 0);
-    ensure_slots((size_t)nci * C.M + C.N, 0);
+    const int nci_cap = (int)cap;
+    C.nci_pending     = true;
+    C.nci             = nci_cap;
+    C.ncj             = nci_cap * C.M / C.N;
+    grow_n(&C.d_ci_natoms, &C.cap_ci_natoms, (size_t)nci_cap + 1, 0);
+    grow_n(&C.d_ci_bin, &C.cap_ci_bin, (size_t)nci_cap + 1, 0);
+    grow_n(&C.d_ci_bbox, &C.cap_ci_bbox, 6 * (size_t)nci_cap + 6, 0);
+    grow_n(&C.d_sv, &C.cap_sv, 3 * (size_t)nci_cap * C.M + 3, 0);
+    ensure_slots((size_t)nci_cap * C.M + C.N, 0);
     if (failed()) return;
     k_cp_fill_ci_bin<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.d_cstart, C.d_ncl, C.d_ci_bin);
-    k_cp_fill_clusters<<<blocks_for(nci * C.M, BLOCK), BLOCK, 0, s>>>(nci, C.M, C.d_ci_bin, C.d_cstart,
-        C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx, C.d_sv, C.d_st,
-        C.d_ci_natoms, C.d_ci_bbox);
+    k_cp_fill_clusters<<<blocks_for((long long)nci_cap * C.M, BLOCK), BLOCK, 0, s>>>(C.d_status + 8,
+        C.M, C.d_ci_bin, C.d_cstart, C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx,
+        C.d_sv, C.d_st, C.d_ci_natoms, C.d_ci_bbox);
     MDG_LAUNCH_CHECK("cp_fill_clusters");
-    atom->Nclusters_local = nci;
+    (void)atom;
 }
 
 void defineJClustersGFX950(Parameter* param, Atom* atom)
 {
     (void)param;
-    if (!ok()) return;
+    (void)atom;
+    if (!queue_ok()) return;
     TimedRegion tr(F_CLUSTER);
-    // ghosts follow: reserve their descriptors later, now the local ones
+    // ghosts follow: reserve their descriptors later, now the local ones (C.ncj is an upper
+    // bound while the cluster count is pending)
     grow_n(&C.d_cj_natoms, &C.cap_cj_natoms, (size_t)C.ncj + 1, 0);
     grow_n(&C.d_cj_bbox, &C.cap_cj_bbox, 6 * (size_t)C.ncj + 6, 0);
     if (failed()) return;
-    k_cp_define_j<<<blocks_for(C.ncj, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.M, C.N, C.d_ci_natoms,
-        C.d_ci_bbox, C.d_cj_natoms, C.d_cj_bbox);
+    if (!C.nci_pending) fill_i32(C.d_status + 8, C.nci, 1);  // the kernel reads the count there
+    k_cp_define_j<<<blocks_for(C.ncj, BLOCK), BLOCK, 0, stream()>>>(C.d_status + 8, C.M, C.N,
+        C.d_ci_natoms, C.d_ci_bbox, C.d_cj_natoms, C.d_cj_bbox);
     MDG_LAUNCH_CHECK("cp_define_j");
-    atom->ncj = C.ncj;
 }
 
 void setupPbcClusterGFX950(Atom* atom, Parameter* param)
 {
     C.sxn_valid = false;
     C.rec_valid = false;
-    if (!ok()) return;
+    if (!queue_ok()) return;
     TimedRegion tr(F_PBC);
     hipStream_t s = stream();
-    const int ncj = C.ncj;
-    grow_n(&C.d_gcnt, &C.cap_gcnt, (size_t)ncj + 1, 0);
-    grow_n(&C.d_goff, &C.cap_goff, (size_t)ncj + 1, 0);
+    const int ncj_cap = C.ncj;  // the count itself, or an upper bound while it is pending
+    grow_n(&C.d_gcnt, &C.cap_gcnt, (size_t)ncj_cap + 1, 0);
+    grow_n(&C.d_goff, &C.cap_goff, (size_t)ncj_cap + 1, 0);
     if (failed()) return;
-    // ghost clusters (status word 2) and ghost atoms (word 3), read in one copy
+    // ghost clusters (status word 2) and ghost atoms (word 3), read in one copy with the
+    // cluster count (word 8) and the binning flags
+    if (!C.nci_pending) fill_i32(C.d_status + 8, C.nci, 1);
     fill_i32(C.d_status + 3, 0, 1);
-    k_cp_ghost_count<<<blocks_for(ncj, GC_T), GC_T, 0, s>>>(ncj, C.d_cj_natoms, C.d_cj_bbox,
-        param->xprd, param->yprd, param->zprd, param->cutneigh, C.d_gcnt, C.d_status + 3);
-    exclusive_scan(C.d_gcnt, C.d_goff, ncj, C.d_status + 2);
+    k_cp_ghost_count<<<blocks_for(ncj_cap, GC_T), GC_T, 0, s>>>(ncj_cap, C.d_status + 8, C.M, C.N,
+        C.d_cj_natoms, C.d_cj_bbox, param->xprd, param->yprd, param->zprd, param->cutneigh,
+        C.d_gcnt, C.d_status + 3);
+    exclusive_scan(C.d_gcnt, C.d_goff, ncj_cap, C.d_status + 2);
     read_status();
     if (failed()) return;
+    apply_cluster_counts();
+    if (failed()) return;
+    const int ncj = C.ncj;
     const int ng = C.h_status[2];
     C.nghost     = ng;
     C.dummy_cj   = ncj + ng;
@@ -2701,6 +2754,8 @@ void setupPbcClusterGFX950(Atom* atom, Parameter* param)
         C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx, C.d_st, C.d_cj_bbox);
     MDG_LAUNCH_CHECK("cp_setup_pbc");
     C.nghost_atoms        = C.h_status[3];
+    atom->Nclusters_local = C.nci;
+    atom->ncj             = C.ncj;
     atom->Nclusters_ghost = ng;
     atom->dummy_cj        = C.dummy_cj;
     atom->Nghost          = C.nghost_atoms;
@@ -3261,6 +3316,7 @@ void energyVirialClusterGFX950(double* epot, double* virial)
 
 void clusterInfoGFX950(int* info)
 {
+    (void)ok();  // reads a pending cluster count back
     int v[11] = { C.nci, C.ncj, C.nghost, C.dummy_cj, C.nghost_atoms, C.g.mbins, C.nstencil,
         C.maxneighs, C.M, C.N, C.natoms };
     memcpy(info, v, sizeof(v));
