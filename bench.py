@@ -45,8 +45,13 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 METRIC = "atom-steps/s (LJ, EAM) + surrogate/plain EAM speedup at fixed force RMSE"
 HBM_PEAK_GBS = 8000.0
 # HIP events bracket every EVENT_EVERY-th launch of the reported kernel inside the timed region
-# (each event pair costs ~5 us of idle GPU between kernels, 2 % of a 1M-atom cluster step)
+# (each event pair costs ~5 us of idle GPU between kernels, 2 % of a 1M-atom cluster step);
+# runs of fewer than 100 steps sample every 5th launch, so that a 20-step run has 4 samples
 EVENT_EVERY = 10
+
+
+def event_every(steps):
+    return EVENT_EVERY if steps >= 100 else 5
 # EAM workloads start from seeded Maxwell-Boltzmann velocities at 1600 K
 MAXWELL_SEED = 1600
 
@@ -131,7 +136,7 @@ def bench_vl(args, mod, resort=0):
         L = mod.lib()
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"force_lj")  # events around the reported kernel only
-        L.mdgfx950TimeEvery(EVENT_EVERY)
+        L.mdgfx950TimeEvery(event_every(args.steps))
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -215,7 +220,7 @@ def bench_cp(args, mod, N=4, half=0):
         L = mod.lib()
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"force_cp")  # events around the reported kernel only
-        L.mdgfx950TimeEvery(EVENT_EVERY)
+        L.mdgfx950TimeEvery(event_every(args.steps))
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -306,7 +311,7 @@ def bench_eam(args, mod, nx=32, resort=0):
         L = mod.lib()
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"force_eam")  # events around the reported kernel only
-        L.mdgfx950TimeEvery(EVENT_EVERY)
+        L.mdgfx950TimeEvery(event_every(args.steps))
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -405,7 +410,7 @@ def bench_ka(args, mod, nx=128, N=4):
         L = mod.lib()
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"force_cp")  # events around the reported kernel only
-        L.mdgfx950TimeEvery(EVENT_EVERY)
+        L.mdgfx950TimeEvery(event_every(args.steps))
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
@@ -483,7 +488,7 @@ def bench_surrogate(args, mod, nx=64):
         sim.run(args.warmup)
         L.mdgfx950Synchronize()
         L.mdgfx950TimeOnly(b"surrogate")  # events around the reported kernel only
-        L.mdgfx950TimeEvery(EVENT_EVERY)
+        L.mdgfx950TimeEvery(event_every(args.steps))
         L.mdgfx950EnableTiming(1)
         t0 = time.perf_counter()
         sim.run(args.steps)
