@@ -826,12 +826,13 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 //   3. the accepted ones in candidate order, ballots giving each its place in the list.
 // The undecided third or so of the candidates then costs one atom-pair pass instead of one per
 // batch in which any lane needed it.
-// Six wavefronts per SIMD (80 VGPRs, a few spilled) ran the 1M-atom build in 442-446 us against
-// 465-469 us at the five the compiler picks by itself (91 VGPRs); at seven (72 VGPRs, more
-// spilled) it took 557 us.
+// 4x4: six wavefronts per SIMD (80 VGPRs, a few spilled) ran the 1M-atom build in 442-446 us
+// against 465-469 us at the five the compiler picks by itself (91 VGPRs); at seven (72 VGPRs,
+// more spilled) it took 557 us.  4x8 (143 VGPRs) forced to six spilled its eight j-positions
+// and doubled its time (667 -> 1258 us); it keeps the compiler's choice.
 constexpr int NB_SEG = 256;
 template <int M, int N>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M == 4 && N == 4) ? 6 : 1))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
