@@ -243,7 +243,9 @@ def bench_cp(args, mod, N=4, half=0):
         # (12 N B); a half list adds the read-modify-write of N j-forces (24 N B) per pair.  The
         # full-list launches of the run loop also integrate (final kick, next half kick and
         # drift): per atom the FP64 velocity read and written and the new position written
-        # (72 B).  (The kernel gathers FP64 positions, 24 B each, to form FP64 displacements.)
+        # (72 B).  (The kernel reads 16-byte FP32 records: cell-relative offsets and the cell
+        # index and type.  The tiled half-list pass sums the j-forces of a tile in LDS and
+        # writes them once per tile; a gather pass adds them per atom and integrates.)
         bytes_launch = nci * (4.0 + 48.0 + 48.0) + pairs * (4.0 + 12.0 * N + (24.0 * N if half else 0.0))
         if not half:
             bytes_launch += 72.0 * natoms
@@ -268,8 +270,8 @@ def bench_cp(args, mod, N=4, half=0):
                                            "over_steps": args.steps}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": ("k_cp_force_f32 (one wavefront per i-cluster, atomic "
-                                    "j-forces)" if half else
+                         "kernel": ("k_cp_force_half_tiled + k_cp_gather_half (j-forces "
+                                    "of a tile summed in LDS, no atomics)" if half else
                                     "k_cp_force_f32_full (four i-clusters per wavefront, one "
                                     "j-atom per lane)"),
                          "avg_launch_ms": avg_ms, "launches": nl.value,
