@@ -45,7 +45,11 @@ void copyAtomsFromClusterGFX950Device(Atom* atom);
 /* replaces setupNeighbor (src/clusterpair/neighbor.c:108-215): 2D bins, stencil */
 void setupNeighborClusterGFX950(Parameter* param, Atom* atom);
 /* replaces buildClustersCPU (neighbor.c:1513-1611) with binAtoms and sortAtomsByZCoord
-   (neighbor.c:1409-1481): same cluster membership and order */
+   (neighbor.c:1409-1481): same cluster membership and order.  The cluster count stays on the
+   device until setupPbcClusterGFX950 reads it with the ghost totals and sets
+   atom->Nclusters_local and atom->ncj (MD-Bench's setup and reneighbour,
+   src/clusterpair/main.c:85-93 and 106-115, read neither in between); the other entry points
+   read a pending count back themselves */
 void buildClustersGFX950(Atom* atom);
 /* replaces defineJClusters (neighbor.c:1807-1927) */
 void defineJClustersGFX950(Parameter* param, Atom* atom);
