@@ -310,6 +310,61 @@ def test_fp32_force_error_vs_fp64_oracle(shape, half, comp):
 
 
 @pytest.mark.parametrize("half", [0, 1])
+def test_fp32_long_cutoff_lists_vs_fp64_oracle(half):
+    """r_c = 4.5 (cutneigh 4.8) on the 8^3 thermalised state: lists about 4 times longer.  The
+    full list outgrows the default 200 entries per i-cluster (the build regrows and rebuilds);
+    the 32-cluster tiles of the half list name more than 2048 entries, more than a tile table
+    holds, so the half list runs the atomic kernel.  Both paths: the device's lists equal the
+    oracle's, forces within the 1e-6 bar of the FP64 oracle, energy and virial within 1e-6."""
+    M, N, nx = 4, 4, 8
+    x, v, t, lat = _thermalised_atoms(nx)
+    sim = product().CPSim(nx=nx, ny=nx, nz=nx, half_neigh=half, cluster_m=M, cluster_n=N,
+                          precision=0, ntimes=0, cutforce=4.5, skin=0.3)
+    try:
+        sim.set_atoms(x, v, t)
+        cp = ob.CP(M, N, (nx * lat,) * 3, 4.8, sim.get("atoms_x"), sim.get("atoms_v"),
+                   sim.get("atoms_type"))
+        cp.reneighbour(half)
+        f64, e64, w64 = cp.force(half, cutf=20.25)
+        np.testing.assert_array_equal(sim.get("neighbors"), cp.get("neighbors"))
+        nn = sim.get("numneigh")
+        if half:
+            tiles = np.add.reduceat(nn, np.arange(0, nn.size, 32))
+            assert tiles.max() > 2048  # a tile outgrows its table: the atomic kernel runs
+        else:
+            assert nn.max() > 200 and sim.info()["maxneighs"] > 200  # the list was regrown
+        live = _live_slots(sim.get("ci_natoms"), M)
+        f32 = sim.get("slot_f").reshape(-1, 3)[live]
+        fr = f64.reshape(-1, 3)[live]
+        rms = np.sqrt(((f32 - fr) ** 2).sum() / (fr ** 2).sum())
+        print("r_c 4.5 half %d: relative RMS force error %.3e" % (half, rms), flush=True)
+        assert rms <= FP32_RMS_TOL
+        e, w = sim.energy_virial()
+        assert abs(e - e64) <= 1e-6 * abs(e64)
+        assert abs(w - w64) <= 1e-6 * abs(w64)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("half", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_energy_and_virial_sums_are_deterministic(half, precision):
+    """Each wavefront stores its energy and virial sums in its own slots and one block adds them
+    in a fixed order: repeated evaluations of one state give the same bits."""
+    M, N, nx = 4, 4, 8
+    x, v, t, lat = _thermalised_atoms(nx)
+    sim = product().CPSim(nx=nx, ny=nx, nz=nx, half_neigh=half, cluster_m=M, cluster_n=N,
+                          precision=precision, ntimes=0)
+    try:
+        sim.set_atoms(x, v, t)
+        first = sim.energy_virial()
+        for _ in range(3):
+            assert sim.energy_virial() == first
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("half", [0, 1])
 def test_fp64_kernel_matches_oracle_on_thermalised_state(half):
     M, N, nx = 4, 4, 8
     x, v, t, lat = _thermalised_atoms(nx)
