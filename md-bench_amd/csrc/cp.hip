@@ -989,7 +989,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
             __builtin_amdgcn_wave_barrier();
         }
     }
-    // single-lane epilogue: masked entries move to the front, in order of arrival
+    // single-lane epilogue: masked entries move to the front, in order of arrival.  In a full
+    // list the only masked entry is the i-cluster's own j-cluster, so for every cluster with
+    // atoms it becomes entry 0: k_cp_force_f32_full tests for the distance-0 pair in its first
+    // iteration only and relies on this (checked by test_own_jcluster_is_entry_zero)
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
         numneigh[ci] = n;
@@ -1014,7 +1017,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
 // time, all M x N pairs each; empty slots sit at far sentinel positions).  The reference then
 // walks the list once: an entry out of range is overwritten by the current last entry, which is
 // tested in its place, and every removal at a position inside the masked prefix shortens that
-// prefix.  One lane replays exactly that walk on the entries staged in LDS.
+// prefix.  One lane replays exactly that walk on the entries staged in LDS.  The i-cluster's
+// own j-cluster (entry 0 of a full list) holds pairs at distance 0, so it is never dropped and
+// never moves: k_cp_force_f32_full's first-iteration-only distance-0 test relies on that.
 constexpr int PRUNE_CAP = 512;  // list entries per i-cluster staged in LDS
 template <int M, int N>
 __global__ void __launch_bounds__(BLOCK) k_cp_prune(int nci, int fma_dist,
