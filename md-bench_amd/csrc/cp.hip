@@ -7,15 +7,20 @@
 // per i-cluster, ordered compaction with ballots, masked entries first).
 //
 // Force kernels:
-//   k_cp_force_f64  one thread per i-slot, the reference kernel's arithmetic (computeForceLJRef,
-//                   src/clusterpair/force_lj.c:22-189); full lists are bit-identical to it.
-//   k_cp_force_f32  one wavefront per i-cluster of 4 atoms: lane = (i-atom 0..3) x (j-slot
-//                   0..15), i.e. four 4-atom or two 8-atom j-clusters per iteration.  The pair
-//                   displacement, r^2 and the cutoff test are FP64 (the FP64 kernel's pair set);
-//                   r^2 and the displacement are rounded once to FP32 and everything after runs
-//                   in FP32 with a Newton-refined v_rcp_f32.  The
-//                   full list is atomic-free; the half list reduces j-forces over the four
-//                   i-lanes with cross-lane shuffles and issues one atomic per j-atom.
+//   k_cp_force_f64          one thread per i-slot, the reference kernel's arithmetic
+//                           (computeForceLJRef, src/clusterpair/force_lj.c:22-189); full lists
+//                           are bit-identical to it.
+//   k_cp_force_f32_full     FP32-mixed, full lists: one wavefront per four i-clusters, one
+//                           j-atom per lane against the group's four i-atoms, j-atoms read as
+//                           16-byte FP32 records (offset in a power-of-two grid cell, cell bytes
+//                           and type), the cutoff decided as in FP64 (a band re-check); with the
+//                           run loop's final kick, next half kick and drift fused in.
+//   k_cp_force_half_tiled   FP32-mixed, half lists: one block per tile of 32 i-clusters, the
+//   + k_cp_gather_half      tile's j-forces summed in LDS as 64-bit fixed point and written once
+//                           per tile, then added per atom in record order (no atomics).
+//   k_cp_force_f32          FP32-mixed, one wavefront per i-cluster of 4 atoms (lane = i-atom x
+//                           j-slot): half lists whose tiles outgrow their tables (one atomic per
+//                           j-atom and component) and compensated half lists.
 #include <type_traits>
 #include <hip/hip_runtime.h>
 
