@@ -1224,6 +1224,16 @@ __device__ inline void kahan_add(float& s, float& c, float t)
     s        = tt;
 }
 
+// Lane i of a 16-lane DPP row reads a lane of the same row chosen by CTRL (a DPP move, no LDS):
+// DPP_XOR1 / DPP_XOR2 (quad permutations: lane i^1, i^2), DPP_HALF_MIRROR (7-i inside each half
+// row), DPP_MIRROR (15-i).
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141;
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
 // (s, c) += (s2, c2) with the rounding error of s + s2 kept (TwoSum).
 __device__ inline void two_sum_merge(float& s, float& c, float s2, float c2)
 {
@@ -1692,23 +1702,35 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         use(A, std::false_type{});
         advance();
     }
-    // reduce over the 16 lanes of each group (lane bits 0..3)
+    // reduce over the 16 lanes of each group (a DPP row)
+    if (!COMP) {
+        // by DPP moves inside the row, no LDS round trips: partners i^1, i^2, then 7-i inside
+        // each half row, then 15-i; every lane of the group ends with the group's sums
+        auto row_step = [&](auto ctrl) {
+            constexpr int C = decltype(ctrl)::value;
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
+            for (int a = 0; a < 4; a++) {
+                fx[a] += dpp_row<C>(fx[a]);
+                fy[a] += dpp_row<C>(fy[a]);
+                fz[a] += dpp_row<C>(fz[a]);
+            }
+        };
+        row_step(std::integral_constant<int, DPP_XOR1>{});
+        row_step(std::integral_constant<int, DPP_XOR2>{});
+        row_step(std::integral_constant<int, DPP_HALF_MIRROR>{});
+        row_step(std::integral_constant<int, DPP_MIRROR>{});
+    } else {
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const float ox = __shfl_xor(fx[a], o, 64), oy = __shfl_xor(fy[a], o, 64),
-                        oz = __shfl_xor(fz[a], o, 64);
-            if (COMP) {
+        for (int o = 1; o < 16; o <<= 1) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const float ox = __shfl_xor(fx[a], o, 64), oy = __shfl_xor(fy[a], o, 64),
+                            oz = __shfl_xor(fz[a], o, 64);
                 const float ocx = __shfl_xor(cx[a], o, 64), ocy = __shfl_xor(cy[a], o, 64),
                             ocz = __shfl_xor(cz[a], o, 64);
                 two_sum_merge(fx[a], cx[a], ox, ocx);
                 two_sum_merge(fy[a], cy[a], oy, ocy);
                 two_sum_merge(fz[a], cz[a], oz, ocz);
-            } else {
-                fx[a] += ox;
-                fy[a] += oy;
-                fz[a] += oz;
             }
         }
     }
@@ -2174,15 +2196,21 @@ __global__ void __launch_bounds__(TILE_WAVES * 64) k_cp_force_half_tiled(int nci
             use(A, false);
             advance();
         }
-        // the i-forces of the round: reduce over the 16 lanes of each group
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1)
+        // the i-forces of the round: reduce over the 16 lanes of each group (a DPP row), as in
+        // k_cp_force_f32_full
+        auto row_step = [&](auto ctrl) {
+            constexpr int C = decltype(ctrl)::value;
 #pragma unroll
             for (int a = 0; a < 4; a++) {
-                fx[a] += __shfl_xor(fx[a], o, 64);
-                fy[a] += __shfl_xor(fy[a], o, 64);
-                fz[a] += __shfl_xor(fz[a], o, 64);
+                fx[a] += dpp_row<C>(fx[a]);
+                fy[a] += dpp_row<C>(fy[a]);
+                fz[a] += dpp_row<C>(fz[a]);
             }
+        };
+        row_step(std::integral_constant<int, DPP_XOR1>{});
+        row_step(std::integral_constant<int, DPP_XOR2>{});
+        row_step(std::integral_constant<int, DPP_HALF_MIRROR>{});
+        row_step(std::integral_constant<int, DPP_MIRROR>{});
         if (live && sub < 4) {
             float rx = 0.f, ry = 0.f, rz = 0.f;
 #pragma unroll
