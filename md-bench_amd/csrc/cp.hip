@@ -852,6 +852,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
     __shared__ int s_cj[BLOCK / 64][NB_SEG];
     __shared__ unsigned char s_fl[BLOCK / 64][NB_SEG];
     __shared__ short s_q[BLOCK / 64][NB_SEG];
+    __shared__ int s_m[BLOCK / 64][NB_SEG];  // the segment's candidates as column entries
     for (int k = threadIdx.x; k < nstencil; k += blockDim.x)
         s_st[k] = stencil[k];
     __syncthreads();
@@ -907,21 +908,22 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
         for (int g0 = 0; g0 < total; g0 += NB_SEG) {
             const int segn = min(NB_SEG, total - g0);
             int nq         = 0;
+            // the segment's candidates: candidate pre + i of this lane's column (i < cnt) is its
+            // entry lo + i; each column lane writes those inside the segment, once, instead of
+            // every candidate searching for its column
+            {
+                const int t1 = min(pre + cnt, g0 + segn);
+                for (int t = max(pre, g0); t < t1; t++)
+                    s_m[w][t - g0] = lo + (t - pre);
+            }
+            // one wavefront: its LDS operations complete in order
+            __builtin_amdgcn_wave_barrier();
             // 1. bounding-box tests
             for (int b0 = 0; b0 < segn; b0 += 64) {
                 const int k = b0 + lane;
-                const int t = g0 + k;
-                // largest column c with pre[c] <= t: that column holds candidate t (columns with
-                // no candidates share their successor's pre and are skipped over)
-                int c = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    const int pc = __shfl(pre, c + step, 64);
-                    if (pc <= t) c += step;
-                }
-                const int m = __shfl(lo, c, 64) + (t - __shfl(pre, c, 64));
                 unsigned char fl = C_REJECT;
                 if (k < segn) {
+                    const int m = s_m[w][k];
                     // the box in column order, loaded beside the entry rather than after it
                     const double* jbb = &jbox[6 * m];
                     double d2         = 0.0;
