@@ -1549,6 +1549,8 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         ti[a] = UNIFORM ? 0 : (int)(w >> 24);
     }
     const int nn = live ? numneigh[ci] : 0;
+    // the group's atom count, for the tail, loaded here rather than after the loop
+    const int nat = live ? ci_natoms[ci] : 0;
     int nmax     = max(nn, __shfl_xor(nn, 16, 64));
     nmax         = max(nmax, __shfl_xor(nmax, 32, 64));
     nmax         = __builtin_amdgcn_readfirstlane(nmax);
@@ -1704,6 +1706,23 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
         use(A, std::false_type{});
         advance();
     }
+    // the tail's loads of this lane's atom (the lanes sub < 4 of a group), issued before the
+    // group sums so that their latency overlaps them
+    const bool tail = sub < 4 && sub < nat;
+    const int si_t  = (live ? ci : 0) * 4 + sub;
+    double px = 0.0, py = 0.0, pz = 0.0, vx0 = 0.0, vy0 = 0.0, vz0 = 0.0;
+    if (tail) {
+        if (xn) {
+            px = sx[3 * si_t];
+            py = sx[3 * si_t + 1];
+            pz = sx[3 * si_t + 2];
+        }
+        if (v) {
+            vx0 = v[3 * si_t];
+            vy0 = v[3 * si_t + 1];
+            vz0 = v[3 * si_t + 2];
+        }
+    }
     // reduce over the 16 lanes of each group (a DPP row)
     if (!COMP) {
         // by DPP moves inside the row, no LDS round trips: partners i^1, i^2, then 7-i inside
@@ -1736,13 +1755,8 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             }
         }
     }
-    // the i-cluster index again, from the wavefront index (a scalar) and the lane id, rather
-    // than kept in registers across the loops
-    const int lane_e = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    const int ci_e   = wv * 4 + (lane_e >> 4);
-    if (ci_e < nci && sub < 4 && sub < ci_natoms[ci_e]) {
+    if (tail) {
         float rx = 0.f, ry = 0.f, rz = 0.f;
-        double px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll
         for (int a = 0; a < 4; a++)
             if (a == sub) {
@@ -1750,20 +1764,15 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
                 ry = COMP ? fy[a] + cy[a] : fy[a];
                 rz = COMP ? fz[a] + cz[a] : fz[a];
             }
-        const int si  = ci_e * 4 + sub;
-        if (xn) {
-            px = sx[3 * si];
-            py = sx[3 * si + 1];
-            pz = sx[3 * si + 2];
-        }
+        const int si  = si_t;
         f[3 * si]     = rx;
         f[3 * si + 1] = ry;
         f[3 * si + 2] = rz;
         if (v) {
             // finalIntegrate (integrate.c:61-86) of this atom, fused: its force is complete
-            double vx = v[3 * si] + dtforce * (double)rx;
-            double vy = v[3 * si + 1] + dtforce * (double)ry;
-            double vz = v[3 * si + 2] + dtforce * (double)rz;
+            double vx = vx0 + dtforce * (double)rx;
+            double vy = vy0 + dtforce * (double)ry;
+            double vz = vz0 + dtforce * (double)rz;
             if (xn) {
                 // and the next step's initialIntegrate (integrate.c:29-59), which kicks with
                 // the same force: half kick, then the drift into the other position buffer,
