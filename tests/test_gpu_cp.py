@@ -231,17 +231,19 @@ def test_fused_force_final_equals_separate_calls():
 
 @pytest.mark.parametrize("shape", [(4, 4), (4, 8)])
 @pytest.mark.parametrize("half", [0, 1])
-def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape, half):
+@pytest.mark.parametrize("every", [(7, 3), (10, 1000)])
+def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape, half, every):
     """The run loop integrates the next step inside the force call
     (computeForceFinalInitialClusterGFX950, two position buffers; for half lists in the gather
     pass) except where a step's full-step velocities are read; a traced run, which reads them
-    every step and so fuses nothing, gives the same trajectory bit for bit, across rebuilds
-    (every 7 steps) and prunes (every 3).  Half lists too: their tile sums are fixed point, so
-    they do not depend on the order in which wavefronts add."""
+    every step and so fuses nothing, gives the same trajectory bit for bit, ghost slots
+    included, across rebuilds (every 7 steps) and prunes (every 3), and over runs of mostly
+    fused steps (rebuilds every 10, no prunes).  Half lists too: their tile sums are fixed
+    point, so they do not depend on the order in which wavefronts add."""
     P = product()
     M, N = shape
-    kw = dict(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=7, prune_every=3,
-              cluster_m=M, cluster_n=N, half_neigh=half)
+    kw = dict(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=every[0],
+              prune_every=every[1], cluster_m=M, cluster_n=N, half_neigh=half)
     out = []
     for trace in (False, True):
         sim = P.CPSim(**kw)
@@ -252,6 +254,28 @@ def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape, ha
             # (forces of empty and ghost slots are never written)
             out.append([sim.get("slot_x"), sim.get("slot_v"),
                         sim.get("slot_f").reshape(-1, 3)[live], sim.get("neighbors")])
+        finally:
+            sim.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("nx", [3, 5])
+def test_fused_run_in_small_boxes_equals_unfused_run(nx):
+    """Boxes so small that a cluster lies within the list cutoff of both faces of an axis (3
+    cells: a cluster has up to 26 ghost images; 5 cells: at most seven): the fused run equals
+    the traced run bit for bit, ghost slots included."""
+    P = product()
+    kw = dict(nx=nx, ny=nx, nz=nx, precision=0, ntimes=0, reneigh_every=10, prune_every=1000,
+              cluster_m=4, cluster_n=4, half_neigh=0)
+    out = []
+    for trace in (False, True):
+        sim = P.CPSim(**kw)
+        try:
+            sim.run(25, trace=trace)
+            live = _live_slots(sim.get("ci_natoms"), 4)
+            out.append([sim.get("slot_x"), sim.get("slot_v"),
+                        sim.get("slot_f").reshape(-1, 3)[live]])
         finally:
             sim.close()
     for a, b in zip(*out):
