@@ -30,6 +30,7 @@
 #include <cstring>
 
 #include "mdgfx950_cp.h"
+#include "block_scan.h"
 #include "runtime.h"
 
 extern "C" int mdgfx950Status(void);  // mdgfx950.h (not includable beside the cluster structs)
@@ -120,6 +121,7 @@ struct CPState {
     float4 *d_sr = nullptr, *d_srn = nullptr;
     size_t cap_sr = 0, cap_srn = 0;
     bool rec_valid = false;  // d_sr matches d_sx in every slot
+    bool cj_from_fill = false;  // the last cluster fill also wrote the j-cluster descriptors
     // half lists without atomics: tile tables and the records' reverse map, built from the
     // current list on the first FP32 force call after a build or prune (tiles_valid); tiles_ok
     // false when a tile outgrew its tables (that list then runs the atomic kernel)
@@ -136,8 +138,6 @@ struct CPState {
     float* d_sf32 = nullptr;
     int* d_st     = nullptr;
     size_t cap_sx = 0, cap_sv = 0, cap_sf64 = 0, cap_sf32 = 0, cap_st = 0;
-    int *d_aoff = nullptr;
-    size_t cap_aoff = 0;
     // ghosts
     int *d_gcnt = nullptr, *d_goff = nullptr, *d_border = nullptr, *d_pbc = nullptr,
         *d_gbin = nullptr;
@@ -270,15 +270,20 @@ __device__ inline unsigned ghost_mask_bb(const double* bb, double xprd, double y
 }
 
 // ---------------------------------------------------------------- updateSingleAtoms + wrap
+// The atoms of cluster ci in the cluster order with the padding dropped start at its bin's first
+// atom plus M per cluster before it in the bin (k_cp_fill_clusters fills only a bin's last
+// cluster partly), from the binning that made the clusters.
 __global__ void k_cp_slots_to_atoms(int nci, int M, const int* __restrict__ ci_natoms,
-    const int* __restrict__ aoff, const double* __restrict__ sx, const double* __restrict__ sv,
-    const int* __restrict__ st, double* __restrict__ ax, double* __restrict__ av,
-    int* __restrict__ at)
+    const int* __restrict__ ci_bin, const int* __restrict__ cstart,
+    const int* __restrict__ abin_start, const double* __restrict__ sx,
+    const double* __restrict__ sv, const int* __restrict__ st, double* __restrict__ ax,
+    double* __restrict__ av, int* __restrict__ at)
 {
     int s  = blockIdx.x * blockDim.x + threadIdx.x;
     int ci = s / M, cii = s % M;
     if (ci >= nci || cii >= ci_natoms[ci]) return;
-    int a = aoff[ci] + cii;
+    const int b = ci_bin[ci];
+    const int a = abin_start[b] + (ci - cstart[b]) * M + cii;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         ax[3 * a + d] = sx[3 * s + d];
@@ -467,68 +472,80 @@ __global__ void __launch_bounds__(SORT_T) k_cp_bin_sort(const int* __restrict__ 
         atoms[s0 + p] = s_idx[p];
 }
 
-__global__ void k_cp_fill_ci_bin(int mbins, const int* __restrict__ cstart, const int* __restrict__ ncl,
-    int* __restrict__ ci_bin)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= mbins) return;
-    for (int k = 0; k < ncl[b]; k++)
-        ci_bin[cstart[b] + k] = b;
-}
+// Scans inside one block of ONE_T threads, each owning a contiguous run of at most ONE_I bins.
+constexpr int ONE_T = 1024, ONE_I = 16;
 
 // buildClustersCPU (neighbor.c:1513-1611): cluster ci holds atoms cl*M .. cl*M+M-1 of its bin.
-// One lane per slot; a cluster's M lanes are adjacent (M divides 64) and reduce its bounding box
-// and atom count with cross-lane reads.
-__global__ void k_cp_fill_clusters(const int* __restrict__ d_nci, int M, const int* __restrict__ ci_bin,
+// One block per bin, one lane per slot of the bin's clusters (contiguous from cstart[b] M on;
+// each wavefront takes 64 of them at a time); a cluster's M lanes are adjacent (M divides 64)
+// and reduce its bounding box and atom count with cross-lane reads.  Also each cluster's bin,
+// and with cj_natoms (M == N) the j-cluster descriptors, which are then the i-clusters'.
+__global__ void k_cp_fill_clusters(int mbins, int M, const int* __restrict__ ncl,
     const int* __restrict__ cstart, const int* __restrict__ abin_start,
     const int* __restrict__ abin_atoms, const double* __restrict__ ax,
     const double* __restrict__ av, const int* __restrict__ at, double* __restrict__ sx,
     double* __restrict__ sv, int* __restrict__ st, int* __restrict__ ci_natoms,
-    double* __restrict__ ci_bbox)
+    double* __restrict__ ci_bbox, int* __restrict__ ci_bin, int* __restrict__ cj_natoms,
+    double* __restrict__ cj_bbox)
 {
-    const int s    = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int ci = s / M, cii = s % M;
-    const bool live = ci < *d_nci;  // the grid covers an upper bound of the count
-    bool has        = false;
-    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-    if (live) {
-        const int b  = ci_bin[ci];
-        const int ac = (ci - cstart[b]) * M + cii;
-        const int s0 = abin_start[b], c = abin_start[b + 1] - s0;
-        has          = ac < c;
-        if (has) {
-            const int i = abin_atoms[s0 + ac];
+    const int b    = blockIdx.x;
+    if (b >= mbins) return;
+    const int c0 = cstart[b], nsl = ncl[b] * M;
+    const int s0 = abin_start[b], c = abin_start[b + 1] - s0;
+    for (int base = threadIdx.x & ~63; base < nsl; base += blockDim.x) {
+        const int k     = base + lane;  // slot within the bin: the atom index within it
+        const bool live = k < nsl;
+        const int s     = c0 * M + k;
+        const int ci = c0 + k / M, cii = k % M;
+        bool has        = false;
+        double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+        if (live) {
+            has = k < c;
+            if (has) {
+                const int i = abin_atoms[s0 + k];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double xv = ax[3 * i + d];
+                    sx[3 * s + d]   = xv;
+                    sv[3 * s + d]   = av[3 * i + d];
+                    lo[d] = hi[d] = xv;
+                }
+                st[s] = at[i];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    sx[3 * s + d] = pad_pos(s, d);
+                    sv[3 * s + d] = 0.0;
+                }
+                st[s] = 0;
+            }
+        }
+        for (int o = 1; o < M; o <<= 1)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const double xv = ax[3 * i + d];
-                sx[3 * s + d]   = xv;
-                sv[3 * s + d]   = av[3 * i + d];
-                lo[d] = hi[d] = xv;
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
             }
-            st[s] = at[i];
-        } else {
+        const unsigned long long bal = __ballot(has);
+        if (live && cii == 0) {
+            const int na  = __popcll((bal >> (lane & ~(M - 1))) & ((M >= 64) ? ~0ull : ((1ull << M) - 1)));
+            ci_natoms[ci] = na;
+            ci_bin[ci]    = b;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                sx[3 * s + d] = pad_pos(s, d);
-                sv[3 * s + d] = 0.0;
+                ci_bbox[6 * ci + 2 * d]     = lo[d];
+                ci_bbox[6 * ci + 2 * d + 1] = hi[d];
             }
-            st[s] = 0;
-        }
-    }
-    for (int o = 1; o < M; o <<= 1)
+            // M == N: the j-clusters are the i-clusters (defineJClusters, fused)
+            if (cj_natoms) {
+                cj_natoms[ci] = na;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
-        }
-    const unsigned long long bal = __ballot(has);
-    if (live && cii == 0) {
-        ci_natoms[ci] = __popcll((bal >> (lane & ~(M - 1))) & ((M >= 64) ? ~0ull : ((1ull << M) - 1)));
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            ci_bbox[6 * ci + 2 * d]     = lo[d];
-            ci_bbox[6 * ci + 2 * d + 1] = hi[d];
+                for (int d = 0; d < 3; d++) {
+                    cj_bbox[6 * ci + 2 * d]     = lo[d];
+                    cj_bbox[6 * ci + 2 * d + 1] = hi[d];
+                }
+            }
         }
     }
 }
@@ -717,6 +734,43 @@ __global__ void k_cp_jbin_count(int mbins, int M, int N, const int* __restrict__
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= mbins) return;
     jcnt[b] = (M == N ? ncl[b] : ncl[b] / 2) + gcnt[b];
+}
+
+// k_cp_jbin_count between the two bin scans it sits between, in one block: the ghost clusters'
+// bin starts (gstart, cursors gcnt cleared for the scatter), the j-cluster bin starts (jstart).
+__global__ void __launch_bounds__(ONE_T) k_cp_jbin_starts(int mbins, int M, int N,
+    const int* __restrict__ ncl, int* __restrict__ gcnt, int* __restrict__ gstart,
+    int* __restrict__ jstart)
+{
+    __shared__ int sh[33];
+    const int per = (mbins + ONE_T - 1) / ONE_T;
+    const int b0  = threadIdx.x * per;
+    int g[ONE_I];
+    int gsum = 0, jsum = 0;
+#pragma unroll
+    for (int k = 0; k < ONE_I; k++) {
+        const bool in = k < per && b0 + k < mbins;
+        g[k]          = in ? gcnt[b0 + k] : 0;
+        gsum += g[k];
+        jsum += in ? (M == N ? ncl[b0 + k] : ncl[b0 + k] / 2) + g[k] : 0;
+    }
+    int gtot, jtot;
+    int goff = mdg::block_excl_scan(gsum, sh, &gtot);
+    int joff = mdg::block_excl_scan(jsum, sh, &jtot);
+#pragma unroll
+    for (int k = 0; k < ONE_I; k++) {
+        if (k < per && b0 + k < mbins) {
+            gstart[b0 + k] = goff;
+            jstart[b0 + k] = joff;
+            gcnt[b0 + k]   = 0;
+            goff += g[k];
+            joff += (M == N ? ncl[b0 + k] : ncl[b0 + k] / 2) + g[k];
+        }
+    }
+    if (threadIdx.x == 0) {
+        gstart[mbins] = gtot;
+        jstart[mbins] = jtot;
+    }
 }
 
 __global__ void k_cp_ghost_scatter(int nghost, const int* __restrict__ gbin,
@@ -2504,7 +2558,7 @@ void freeDeviceClusterGFX950(void)
     void* ptrs[] = { C.d_stencil, C.d_ax, C.d_av, C.d_at, C.d_abin_of, C.d_abin_cnt,
         C.d_abin_start, C.d_abin_atoms, C.d_ncl, C.d_cstart, C.d_ci_natoms, C.d_ci_bin,
         C.d_cj_natoms, C.d_ci_bbox, C.d_cj_bbox, C.d_sx, C.d_sv, C.d_sf64, C.d_sf32, C.d_st,
-        C.d_aoff, C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
+        C.d_gcnt, C.d_goff, C.d_border, C.d_pbc, C.d_gbin, C.d_jcnt, C.d_jstart,
         C.d_jlist, C.d_jkey, C.d_zidx, C.d_gcur, C.d_gbin_start, C.d_gbin_list, C.d_nb,
         C.d_numneigh, C.d_nmasked, C.d_red, C.d_ev, C.d_evp, C.d_status, C.d_sxn, C.d_sr, C.d_srn,
         C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_rec_off, C.d_rcount, C.d_rstart, C.d_rcur,
@@ -2636,13 +2690,10 @@ void updateSingleAtomsGFX950(Parameter* param, Atom* atom)
     (void)param;
     if (!ok() || C.nci == 0) return;
     TimedRegion tr(F_CLUSTER);
-    grow_n(&C.d_aoff, &C.cap_aoff, (size_t)C.nci + 1, 0);
-    if (failed()) return;
     hipStream_t s = stream();
-    exclusive_scan(C.d_ci_natoms, C.d_aoff, C.nci, C.d_status + 4);
     int ns = C.nci * C.M;
-    k_cp_slots_to_atoms<<<blocks_for(ns, BLOCK), BLOCK, 0, s>>>(C.nci, C.M, C.d_ci_natoms, C.d_aoff,
-        C.d_sx, C.d_sv, C.d_st, C.d_ax, C.d_av, C.d_at);
+    k_cp_slots_to_atoms<<<blocks_for(ns, BLOCK), BLOCK, 0, s>>>(C.nci, C.M, C.d_ci_natoms,
+        C.d_ci_bin, C.d_cstart, C.d_abin_start, C.d_sx, C.d_sv, C.d_st, C.d_ax, C.d_av, C.d_at);
     MDG_LAUNCH_CHECK("slots_to_atoms");
     (void)atom;
 }
@@ -2710,38 +2761,44 @@ void buildClustersGFX950(Atom* atom)
     grow_n(&C.d_ncl, &C.cap_ncl, (size_t)mb + 1, 0);
     grow_n(&C.d_cstart, &C.cap_cstart, (size_t)mb + 1, 0);
     if (failed()) return;
-    fill_i32(C.d_status, 0, 8);
-    fill_i32(C.d_abin_cnt, 0, (size_t)mb + 1);
-    k_cp_bin_count<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_ax, C.g, C.d_abin_of,
-        C.d_abin_cnt, C.d_status);
-    exclusive_scan(C.d_abin_cnt, C.d_abin_start, mb, C.d_abin_start + mb);
-    fill_i32(C.d_abin_cnt, 0, (size_t)mb);
-    k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
-        C.d_abin_cnt, C.d_abin_atoms);
-    k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
-        C.d_status);
-    // the cluster count to status word 8
-    exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb, C.d_status + 8);
-    MDG_LAUNCH_CHECK("cp_binning");
     const long long cap = ((long long)na + C.M - 1) / C.M + (long long)mb * (C.N > C.M ? 2 : 1);
     if (cap * C.M + C.N >= (1ll << 31)) {
         set_error(E_ARG, "cluster binning: %lld cluster slots exceed 32-bit indexing", cap * C.M);
         return;
     }
     const int nci_cap = (int)cap;
-    C.nci_pending     = true;
-    C.nci             = nci_cap;
-    C.ncj             = nci_cap * C.M / C.N;
     grow_n(&C.d_ci_natoms, &C.cap_ci_natoms, (size_t)nci_cap + 1, 0);
     grow_n(&C.d_ci_bin, &C.cap_ci_bin, (size_t)nci_cap + 1, 0);
     grow_n(&C.d_ci_bbox, &C.cap_ci_bbox, 6 * (size_t)nci_cap + 6, 0);
     grow_n(&C.d_sv, &C.cap_sv, 3 * (size_t)nci_cap * C.M + 3, 0);
     ensure_slots((size_t)nci_cap * C.M + C.N, 0);
+    // M == N: the j-cluster descriptors are written with the i-clusters' (defineJClusters
+    // then has nothing to do); the ghosts' follow after them
+    C.cj_from_fill = C.M == C.N;
+    if (C.cj_from_fill) {
+        grow_n(&C.d_cj_natoms, &C.cap_cj_natoms, (size_t)nci_cap + 1, 0);
+        grow_n(&C.d_cj_bbox, &C.cap_cj_bbox, 6 * (size_t)nci_cap + 6, 0);
+    }
     if (failed()) return;
-    k_cp_fill_ci_bin<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.d_cstart, C.d_ncl, C.d_ci_bin);
-    k_cp_fill_clusters<<<blocks_for((long long)nci_cap * C.M, BLOCK), BLOCK, 0, s>>>(C.d_status + 8,
-        C.M, C.d_ci_bin, C.d_cstart, C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx,
-        C.d_sv, C.d_st, C.d_ci_natoms, C.d_ci_bbox);
+    fill_i32_multi({ { C.d_status, 0, 8 }, { C.d_abin_cnt, 0, (size_t)mb + 1 } });
+    k_cp_bin_count<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_ax, C.g, C.d_abin_of,
+        C.d_abin_cnt, C.d_status);
+    // the bin starts; the counts cleared for the scatter's cursors
+    exclusive_scan_take(C.d_abin_cnt, C.d_abin_start, mb, C.d_abin_start + mb);
+    k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
+        C.d_abin_cnt, C.d_abin_atoms);
+    k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
+        C.d_status);
+    // the cluster starts and the cluster count (to status word 8)
+    exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb, C.d_status + 8);
+    MDG_LAUNCH_CHECK("cp_binning");
+    C.nci_pending = true;
+    C.nci         = nci_cap;
+    C.ncj         = nci_cap * C.M / C.N;
+    k_cp_fill_clusters<<<mb, BLOCK, 0, s>>>(mb, C.M, C.d_ncl, C.d_cstart,
+        C.d_abin_start, C.d_abin_atoms, C.d_ax, C.d_av, C.d_at, C.d_sx, C.d_sv, C.d_st,
+        C.d_ci_natoms, C.d_ci_bbox, C.d_ci_bin, C.cj_from_fill ? C.d_cj_natoms : nullptr,
+        C.d_cj_bbox);
     MDG_LAUNCH_CHECK("cp_fill_clusters");
     (void)atom;
 }
@@ -2751,6 +2808,10 @@ void defineJClustersGFX950(Parameter* param, Atom* atom)
     (void)param;
     (void)atom;
     if (!queue_ok()) return;
+    if (C.cj_from_fill) {  // written by the cluster fill (M == N)
+        C.cj_from_fill = false;
+        return;
+    }
     TimedRegion tr(F_CLUSTER);
     // ghosts follow: reserve their descriptors later, now the local ones (C.ncj is an upper
     // bound while the cluster count is pending)
@@ -2850,14 +2911,19 @@ void binJClustersGFX950(Parameter* param, Atom* atom)
     grow_n(&C.d_jlist, &C.cap_jlist, (size_t)C.ncj + ng + 1, 0);
     grow_n(&C.d_jkey, &C.cap_jkey, 6 * ((size_t)C.ncj + ng + 1), 0);
     if (failed()) return;
-    fill_i32(C.d_status, 0, 1);
-    fill_i32(C.d_gcur, 0, (size_t)mb + 1);
+    fill_i32_multi({ { C.d_status, 0, 1 }, { C.d_gcur, 0, (size_t)mb + 1 } });
     k_cp_ghost_bin<<<blocks_for(ng, BLOCK), BLOCK, 0, s>>>(C.ncj, ng, C.N, C.g, C.d_cj_natoms,
         C.d_sx, C.d_pbc, C.d_gbin, C.d_gcur, C.d_status);
-    exclusive_scan(C.d_gcur, C.d_gbin_start, mb, C.d_gbin_start + mb);
-    k_cp_jbin_count<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.M, C.N, C.d_ncl, C.d_gcur, C.d_jcnt);
-    exclusive_scan(C.d_jcnt, C.d_jstart, mb, C.d_jstart + mb);
-    fill_i32(C.d_gcur, 0, (size_t)mb + 1);
+    if (mb <= ONE_T * ONE_I) {
+        k_cp_jbin_starts<<<1, ONE_T, 0, s>>>(mb, C.M, C.N, C.d_ncl, C.d_gcur, C.d_gbin_start,
+            C.d_jstart);
+    } else {
+        exclusive_scan(C.d_gcur, C.d_gbin_start, mb, C.d_gbin_start + mb);
+        k_cp_jbin_count<<<blocks_for(mb, BLOCK), BLOCK, 0, s>>>(mb, C.M, C.N, C.d_ncl, C.d_gcur,
+            C.d_jcnt);
+        exclusive_scan(C.d_jcnt, C.d_jstart, mb, C.d_jstart + mb);
+        fill_i32(C.d_gcur, 0, (size_t)mb + 1);
+    }
     k_cp_ghost_scatter<<<blocks_for(ng, BLOCK), BLOCK, 0, s>>>(ng, C.d_gbin, C.d_gbin_start,
         C.d_gcur, C.d_gbin_list);
     k_cp_jbin_merge<<<mb, 64, 0, s>>>(C.M, C.N, C.ncj, C.d_ncl, C.d_cstart, C.d_gbin_start,
@@ -2892,7 +2958,8 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     // largest j-cluster height (status words 12-13 hold it as a 64-bit pattern)
     static_assert(ST_REC_WORD != 12 && ST_REC_WORD != 13, "status words 12..13 hold hmax");
     unsigned long long* d_hmax = reinterpret_cast<unsigned long long*>(C.d_status + 12);
-    fill_i32(C.d_status + 12, 0, 2);
+    // hmax and the first attempt's flag and count words
+    fill_i32_multi({ { C.d_status + 12, 0, 2 }, { C.d_status, 0, 2 } });
     const int ncj_all = C.ncj + C.nghost;
     if (ncj_all > 0)
         k_cp_jheight<<<std::min(blocks_for(ncj_all, BLOCK), 512u), BLOCK, 0, s>>>(ncj_all,
@@ -2902,7 +2969,7 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
         grow_n(&C.d_numneigh, &C.cap_numneigh, (size_t)nci + 1, 0);
         grow_n(&C.d_nmasked, &C.cap_nmasked, (size_t)nci + 1, 0);
         if (failed()) return;
-        fill_i32(C.d_status, 0, 2);
+        if (attempt > 0) fill_i32(C.d_status, 0, 2);
         unsigned grid = blocks_for(nci, BLOCK / 64);
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \

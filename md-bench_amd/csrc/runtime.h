@@ -1,6 +1,7 @@
 // Internal runtime shared by the gfx950 translation units: sticky status, the library stream,
 // event-based kernel timing, device buffers and a three-pass exclusive scan.
 #pragma once
+#include <initializer_list>
 
 #include <hip/hip_runtime.h>
 
@@ -65,10 +66,24 @@ void dfree(void* p);
 void fill_i32(int* p, int v, size_t n);
 void fill_f64(double* p, double v, size_t n);
 void fill_f32(float* p, float v, size_t n);
+// Up to FILL_MULTI_MAX int ranges filled in one launch.
+constexpr int FILL_MULTI_MAX = 4;
+struct FillRange {
+    int* p;
+    int v;
+    size_t n;
+};
+struct FillRanges {
+    FillRange r[FILL_MULTI_MAX];
+    int k;
+};
+void fill_i32_multi(std::initializer_list<FillRange> ranges);
 
 // Exclusive scan of n ints into out (in == out allowed); writes the total to *d_total (device).
 // out[i] = in[0] + ... + in[i-1]; the total goes to *d_total and, if given, *d_total2.
 void exclusive_scan(const int* in, int* out, int n, int* d_total, int* d_total2 = nullptr);
+// The same (in and out distinct, one total) and in[0 .. n) cleared to 0 afterwards.
+void exclusive_scan_take(int* in, int* out, int n, int* d_total);
 
 // Grid size for n items at `block` threads per block.
 inline unsigned blocks_for(long long n, int block)

@@ -10,6 +10,7 @@
 
 #include "mdgfx950.h"
 #include "runtime.h"
+#include "block_scan.h"
 
 namespace mdg {
 
@@ -165,6 +166,36 @@ static void fill_t(T* p, T v, size_t n)
     MDG_LAUNCH_CHECK("fill");
 }
 void fill_i32(int* p, int v, size_t n) { fill_t<int>(p, v, n); }
+
+// several int ranges in one launch (each range is one kernel boundary fewer)
+__global__ void k_fill_multi(FillRanges f)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int q = 0; q < f.k; q++)
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.r[q].n; i += stride)
+            f.r[q].p[i] = f.r[q].v;
+}
+
+void fill_i32_multi(std::initializer_list<FillRange> ranges)
+{
+    if (failed()) return;
+    FillRanges f = {};
+    size_t nmax  = 0;
+    for (const FillRange& r : ranges) {
+        if (f.k == FILL_MULTI_MAX) {
+            set_error(1, "fill_i32_multi: more than %d ranges", FILL_MULTI_MAX);
+            return;
+        }
+        if (r.n == 0) continue;
+        f.r[f.k++] = r;
+        if (r.n > nmax) nmax = r.n;
+    }
+    if (f.k == 0) return;
+    unsigned g = blocks_for((long long)nmax, 256);
+    if (g > 8192) g = 8192;
+    k_fill_multi<<<g, 256, 0, stream()>>>(f);
+    MDG_LAUNCH_CHECK("fill_multi");
+}
 void fill_f64(double* p, double v, size_t n) { fill_t<double>(p, v, n); }
 void fill_f32(float* p, float v, size_t n) { fill_t<float>(p, v, n); }
 
@@ -172,37 +203,6 @@ void fill_f32(float* p, float v, size_t n) { fill_t<float>(p, v, n); }
 // Tiles of 256 threads x 8 items.  Pass 1 sums tiles, pass 2 scans tile sums in one block,
 // pass 3 rescans each tile with its offset.
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
-
-__device__ inline int wave_incl_scan(int v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// Exclusive scan of one value per thread across the block; returns the block total in *tot.
-__device__ inline int block_excl_scan(int v, int* sh, int* tot)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int inc = wave_incl_scan(v);
-    if (lane == 63) sh[w] = inc;
-    __syncthreads();
-    if (w == 0) {
-        int s  = lane < nw ? sh[lane] : 0;
-        int si = wave_incl_scan(s);
-        if (lane < nw) sh[lane] = si - s;
-        if (lane == nw - 1) sh[32] = si;
-    }
-    __syncthreads();
-    int r = inc - v + sh[w];
-    *tot  = sh[32];
-    __syncthreads();
-    return r;
-}
 
 __global__ void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ tile_sums)
 {
@@ -287,6 +287,34 @@ __global__ void __launch_bounds__(SCAN1_T) k_scan_single(const int* in, int* out
     if (threadIdx.x == 0 && d_total2) *d_total2 = tot;
 }
 
+// k_scan_single that also clears its input (each value is read and cleared by the one thread
+// that owns it); in and out must not alias.
+__global__ void __launch_bounds__(SCAN1_T) k_scan_single_take(int* __restrict__ in,
+    int* __restrict__ out, int n, int* __restrict__ d_total)
+{
+    __shared__ int sh[33];
+    const int per = (n + SCAN1_T - 1) / SCAN1_T;
+    const int b   = threadIdx.x * per;
+    int v[SCAN1_I];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_I; k++) {
+        v[k] = (k < per && b + k < n) ? in[b + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int off = block_excl_scan(s, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < SCAN1_I; k++) {
+        if (k < per && b + k < n) {
+            out[b + k] = off;
+            in[b + k]  = 0;
+        }
+        off += v[k];
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = tot;
+}
+
 static int* g_tiles       = nullptr;
 static size_t g_tiles_cap = 0;
 
@@ -306,6 +334,18 @@ void exclusive_scan(const int* in, int* out, int n, int* d_total, int* d_total2)
     k_scan_tiles<<<1, 1024, 0, s>>>(g_tiles, ntiles, d_total, d_total2);
     k_scan_apply<<<ntiles, SCAN_T, 0, s>>>(in, out, n, g_tiles);
     MDG_LAUNCH_CHECK("exclusive_scan");
+}
+
+void exclusive_scan_take(int* in, int* out, int n, int* d_total)
+{
+    if (failed()) return;
+    if (n <= SCAN1_T * SCAN1_I) {
+        k_scan_single_take<<<1, SCAN1_T, 0, stream()>>>(in, out, n, d_total);
+        MDG_LAUNCH_CHECK("exclusive_scan_take");
+        return;
+    }
+    exclusive_scan(in, out, n, d_total);
+    fill_i32(in, 0, (size_t)n);
 }
 
 }  // namespace mdg
