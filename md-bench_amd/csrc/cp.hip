@@ -122,6 +122,9 @@ struct CPState {
     size_t cap_sr = 0, cap_srn = 0;
     bool rec_valid = false;  // d_sr matches d_sx in every slot
     bool cj_from_fill = false;  // the last cluster fill also wrote the j-cluster descriptors
+    // updateSingleAtoms / updateAtomsPbc requested and not yet run (see flush_atoms)
+    bool atoms_copy_pending = false, atoms_wrap_pending = false;
+    double wrap_prd[3] = { 0.0, 0.0, 0.0 };
     // half lists without atomics: tile tables and the records' reverse map, built from the
     // current list on the first FP32 force call after a build or prune (tiles_valid); tiles_ok
     // false when a tile outgrew its tables (that list then runs the atomic kernel)
@@ -273,23 +276,58 @@ __device__ inline unsigned ghost_mask_bb(const double* bb, double xprd, double y
 // The atoms of cluster ci in the cluster order with the padding dropped start at its bin's first
 // atom plus M per cluster before it in the bin (k_cp_fill_clusters fills only a bin's last
 // cluster partly), from the binning that made the clusters.
+// With wrap, each position is also wrapped into the box as k_cp_wrap does (updateAtomsPbc); with
+// cnt, each atom is also binned and counted as k_cp_bin_count does (the start of the next
+// binning), the counts summed per wavefront and bin before the atomic.
 __global__ void k_cp_slots_to_atoms(int nci, int M, const int* __restrict__ ci_natoms,
     const int* __restrict__ ci_bin, const int* __restrict__ cstart,
     const int* __restrict__ abin_start, const double* __restrict__ sx,
     const double* __restrict__ sv, const int* __restrict__ st, double* __restrict__ ax,
-    double* __restrict__ av, int* __restrict__ at)
+    double* __restrict__ av, int* __restrict__ at, int wrap, double xprd, double yprd,
+    double zprd, CGrid g, int* __restrict__ bin_of, int* __restrict__ cnt,
+    int* __restrict__ status)
 {
-    int s  = blockIdx.x * blockDim.x + threadIdx.x;
-    int ci = s / M, cii = s % M;
-    if (ci >= nci || cii >= ci_natoms[ci]) return;
-    const int b = ci_bin[ci];
-    const int a = abin_start[b] + (ci - cstart[b]) * M + cii;
+    const int s    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int ci = s / M, cii = s % M;
+    int bn       = -1;
+    if (ci < nci && cii < ci_natoms[ci]) {
+        const int b         = ci_bin[ci];
+        const int a         = abin_start[b] + (ci - cstart[b]) * M + cii;
+        const double prd[3] = { xprd, yprd, zprd };
+        double x[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        ax[3 * a + d] = sx[3 * s + d];
-        av[3 * a + d] = sv[3 * s + d];
+        for (int d = 0; d < 3; d++) {
+            double xx = sx[3 * s + d];
+            if (wrap) {
+                if (xx < 0.0) xx += prd[d];
+                else if (xx >= prd[d]) xx -= prd[d];
+            }
+            x[d]          = xx;
+            ax[3 * a + d] = xx;
+            av[3 * a + d] = sv[3 * s + d];
+        }
+        at[a] = st[s];
+        if (cnt) {
+            int ix, iy;
+            c2b2d(g, x[0], x[1], &ix, &iy);
+            bn = iy * g.mbinx + ix + 1;
+            if (bn < 0 || bn >= g.mbins) {
+                atomicOr(status, ST_BIN_RANGE);
+                bn = -1;
+            }
+            bin_of[a] = bn;
+        }
     }
-    at[a] = st[s];
+    if (!cnt) return;  // uniform
+    unsigned long long act = __ballot(bn >= 0);
+    while (act) {
+        const int l                  = __ffsll((long long)act) - 1;
+        const int lb                 = __shfl(bn, l, 64);
+        const unsigned long long grp = __ballot(bn == lb);
+        if (lane == l) atomicAdd(&cnt[lb], __popcll(grp));
+        act &= ~grp;
+    }
 }
 
 __global__ void k_cp_wrap(int n3, double* __restrict__ x, double xprd, double yprd, double zprd)
@@ -785,7 +823,11 @@ __global__ void k_cp_ghost_scatter(int nghost, const int* __restrict__ gbin,
 
 // One wavefront per bin: locals (in cluster order) then ghosts, stably ordered by bbminz --
 // the order the reference's sorted insertion of ghosts produces.  Ties go to the smaller
-// j-cluster index (locals before ghosts, ghosts in creation order).
+// j-cluster index (locals before ghosts, ghosts in creation order).  A bin's local clusters
+// take its z-sorted atoms in order, so their keys (their first atoms' z) come sorted: each
+// local's place is then its own index plus the ghosts below it, and a ghost's the locals at or
+// below it plus the ghosts before it (interior bins, without ghosts, are a copy).  Unsorted
+// locals (checked) fall back to counting every entry below each entry.
 __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
     const int* __restrict__ ncl, const int* __restrict__ cstart, const int* __restrict__ gstart,
     const int* __restrict__ glist, const double* __restrict__ cj_bbox,
@@ -809,12 +851,28 @@ __global__ void __launch_bounds__(64) k_cp_jbin_merge(int M, int N, int ncj,
         s_key[e]  = cj_bbox[6 * cj + 4];
     }
     __syncthreads();
+    bool unsorted = false;
+    for (int e = lane; e + 1 < nloc; e += 64)
+        unsorted = unsorted || !(s_key[e] <= s_key[e + 1]);
+    unsorted = __ballot(unsorted) != 0;
     const int out = jstart[b];
     for (int e = lane; e < n; e += 64) {
         double k = s_key[e];
         int cj   = s_cj[e], r = 0;
-        for (int f = 0; f < n; f++)
-            r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
+        if (unsorted) {
+            for (int f = 0; f < n; f++)
+                r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
+        } else if (e < nloc) {
+            // locals before ghosts on equal keys (smaller index)
+            r = e;
+            for (int f = nloc; f < n; f++)
+                r += s_key[f] < k;
+        } else {
+            for (int f = 0; f < nloc; f++)
+                r += s_key[f] <= k;
+            for (int f = nloc; f < n; f++)
+                r += (s_key[f] < k) || (s_key[f] == k && s_cj[f] < cj);
+        }
         jlist[out + r] = cj;
 #pragma unroll
         for (int q = 0; q < 6; q++)
@@ -2577,9 +2635,14 @@ void freeDeviceClusterGFX950(void)
     C.comp = comp;
 }
 
+namespace {
+void flush_atoms();
+}
+
 void copyAtomsToClusterGFX950Device(Atom* atom)
 {
     if (!ok()) return;
+    flush_atoms();  // requested before this upload, so run before it
     size_t n = (size_t)atom->Nlocal;
     C.natoms = atom->Nlocal;
     grow_n(&C.d_ax, &C.cap_ax, 3 * n + 3, 0);
@@ -2602,6 +2665,7 @@ void copyAtomsToClusterGFX950Device(Atom* atom)
 void copyAtomsFromClusterGFX950Device(Atom* atom)
 {
     if (!ok()) return;
+    flush_atoms();
     size_t n = (size_t)C.natoms;
     double* v = (double*)malloc(sizeof(double) * 3 * n + 8);
     MDG_HIP(hipMemcpyAsync(atom->x, C.d_ax, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -2685,17 +2749,43 @@ void setupNeighborClusterGFX950(Parameter* param, Atom* atom)
     MDG_HIP(hipStreamSynchronize(stream()));
 }
 
+// updateSingleAtoms and updateAtomsPbc are deferred: the next binning does both in its first
+// kernel (one pass over the slots: copy, wrap, bin count); anything else that reads or writes
+// the atom arrays first runs what is pending (flush_atoms).
+namespace {
+void launch_slots_to_atoms(bool wrap, bool count)
+{
+    const int ns = C.nci * C.M;
+    k_cp_slots_to_atoms<<<blocks_for(ns, BLOCK), BLOCK, 0, stream()>>>(C.nci, C.M,
+        C.d_ci_natoms, C.d_ci_bin, C.d_cstart, C.d_abin_start, C.d_sx, C.d_sv, C.d_st, C.d_ax,
+        C.d_av, C.d_at, wrap ? 1 : 0, C.wrap_prd[0], C.wrap_prd[1], C.wrap_prd[2], C.g,
+        count ? C.d_abin_of : nullptr, count ? C.d_abin_cnt : nullptr, C.d_status);
+    MDG_LAUNCH_CHECK("slots_to_atoms");
+}
+
+void flush_atoms()
+{
+    const bool copy = C.atoms_copy_pending, wrap = C.atoms_wrap_pending;
+    C.atoms_copy_pending = C.atoms_wrap_pending = false;
+    if (failed() || !(copy || wrap)) return;
+    if (copy) {
+        launch_slots_to_atoms(wrap, false);
+    } else {
+        const int n3 = 3 * C.natoms;
+        k_cp_wrap<<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.d_ax, C.wrap_prd[0],
+            C.wrap_prd[1], C.wrap_prd[2]);
+        MDG_LAUNCH_CHECK("cp_wrap");
+    }
+}
+}  // namespace
+
 void updateSingleAtomsGFX950(Parameter* param, Atom* atom)
 {
     (void)param;
-    if (!ok() || C.nci == 0) return;
-    TimedRegion tr(F_CLUSTER);
-    hipStream_t s = stream();
-    int ns = C.nci * C.M;
-    k_cp_slots_to_atoms<<<blocks_for(ns, BLOCK), BLOCK, 0, s>>>(C.nci, C.M, C.d_ci_natoms,
-        C.d_ci_bin, C.d_cstart, C.d_abin_start, C.d_sx, C.d_sv, C.d_st, C.d_ax, C.d_av, C.d_at);
-    MDG_LAUNCH_CHECK("slots_to_atoms");
     (void)atom;
+    if (!ok() || C.nci == 0) return;
+    flush_atoms();  // a wrap requested before this copy runs before it
+    C.atoms_copy_pending = true;
 }
 
 void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy)
@@ -2703,11 +2793,11 @@ void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy)
     (void)dummy;
     (void)atom;
     if (!ok()) return;
-    TimedRegion tr(F_PBC);
-    int n3 = 3 * C.natoms;
-    k_cp_wrap<<<blocks_for(n3, BLOCK), BLOCK, 0, stream()>>>(n3, C.d_ax, param->xprd, param->yprd,
-        param->zprd);
-    MDG_LAUNCH_CHECK("cp_wrap");
+    if (C.atoms_wrap_pending) flush_atoms();
+    C.atoms_wrap_pending = true;
+    C.wrap_prd[0]        = param->xprd;
+    C.wrap_prd[1]        = param->yprd;
+    C.wrap_prd[2]        = param->zprd;
 }
 
 namespace {
@@ -2756,6 +2846,20 @@ void buildClustersGFX950(Atom* atom)
     const int na = C.natoms, mb = C.g.mbins;
     grow_n(&C.d_abin_of, &C.cap_abin_of, (size_t)na + 1, 0);
     grow_n(&C.d_abin_cnt, &C.cap_abin_cnt, (size_t)mb + 1, 0);
+    if (failed()) return;
+    fill_i32_multi({ { C.d_status, 0, 8 }, { C.d_abin_cnt, 0, (size_t)mb + 1 } });
+    if (C.atoms_copy_pending) {
+        // the deferred updateSingleAtoms (and updateAtomsPbc) with the bin count, in one pass
+        // over the slots; launched before anything below can move the slots or the last
+        // binning's arrays it reads
+        const bool wrap      = C.atoms_wrap_pending;
+        C.atoms_copy_pending = C.atoms_wrap_pending = false;
+        launch_slots_to_atoms(wrap, true);
+    } else {
+        flush_atoms();
+        k_cp_bin_count<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_ax, C.g, C.d_abin_of,
+            C.d_abin_cnt, C.d_status);
+    }
     grow_n(&C.d_abin_start, &C.cap_abin_start, (size_t)mb + 1, 0);
     grow_n(&C.d_abin_atoms, &C.cap_abin_atoms, (size_t)na + 1, 0);
     grow_n(&C.d_ncl, &C.cap_ncl, (size_t)mb + 1, 0);
@@ -2780,9 +2884,6 @@ void buildClustersGFX950(Atom* atom)
         grow_n(&C.d_cj_bbox, &C.cap_cj_bbox, 6 * (size_t)nci_cap + 6, 0);
     }
     if (failed()) return;
-    fill_i32_multi({ { C.d_status, 0, 8 }, { C.d_abin_cnt, 0, (size_t)mb + 1 } });
-    k_cp_bin_count<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_ax, C.g, C.d_abin_of,
-        C.d_abin_cnt, C.d_status);
     // the bin starts; the counts cleared for the scatter's cursors
     exclusive_scan_take(C.d_abin_cnt, C.d_abin_start, mb, C.d_abin_start + mb);
     k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
@@ -3163,6 +3264,7 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
     bool fuse_initial, double* secs)
 {
     if (!ok()) return 0;
+    flush_atoms();  // a pending copy reads the slots before this call moves them
     const double t0 = wtime();
     int fused       = 0;
     hipStream_t s   = stream();
@@ -3336,6 +3438,7 @@ void initialIntegrateClusterGFX950(Parameter* param, Atom* atom)
 {
     (void)atom;
     if (!ok()) return;
+    flush_atoms();
     TimedRegion tr(F_INTEGRATE);
     int n3 = 3 * C.nci * C.M;
     if (!(C.forces_are_f32 ? (void*)C.d_sf32 : (void*)C.d_sf64)) {
@@ -3356,6 +3459,7 @@ void finalIntegrateClusterGFX950(Parameter* param, Atom* atom)
 {
     (void)atom;
     if (!ok()) return;
+    flush_atoms();
     TimedRegion tr(F_INTEGRATE);
     int n3 = 3 * C.nci * C.M;
     if (!(C.forces_are_f32 ? (void*)C.d_sf32 : (void*)C.d_sf64)) {
@@ -3442,6 +3546,7 @@ void clusterInfoGFX950(int* info)
 long long uploadClusterArrayGFX950(const char* name, const void* in, long long n)
 {
     if (!ok()) return -3;
+    flush_atoms();  // a pending copy reads the slots as they were
     void* dst       = nullptr;
     long long avail = 0;
     if (!strcmp(name, "slot_x")) { dst = C.d_sx; avail = 3LL * (long long)nslots_all(); }
@@ -3457,6 +3562,7 @@ long long uploadClusterArrayGFX950(const char* name, const void* in, long long n
 long long downloadClusterArrayGFX950(const char* name, void* out, long long cap)
 {
     if (!ok()) return -3;
+    flush_atoms();  // the atom arrays as the calls so far leave them
     hipStream_t s = stream();
     MDG_HIP(hipStreamSynchronize(s));
     const size_t nsl = nslots_all(), nj = (size_t)C.ncj + C.nghost;
