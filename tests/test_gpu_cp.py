@@ -260,6 +260,41 @@ def test_fused_integration_over_rebuilds_and_prunes_equals_unfused_run(shape, ha
         np.testing.assert_array_equal(a, b)
 
 
+def test_deferred_atom_updates_read_like_eager_calls():
+    """updateSingleAtoms and updateAtomsPbc run inside the next binning's first kernel; whatever
+    reads the atoms before that sees them as the two calls leave them: the slots' atoms in
+    cluster order, then wrapped into the box (each call on its own, and the two together)."""
+    import ctypes as C
+    P = product()
+    L = P.lib()
+    L.updateSingleAtomsGFX950.argtypes = [C.c_void_p, C.c_void_p]
+    L.updateAtomsPbcClusterGFX950.argtypes = [C.c_void_p, C.c_void_p, C.c_bool]
+    sim = P.CPSim(nx=8, ny=8, nz=8, precision=0, ntimes=0, reneigh_every=1000, cluster_m=4,
+                  cluster_n=4)
+    try:
+        sim.run(30)  # atoms drift across the faces (no rebuild, so no wrap)
+        prm = P.Parameter.from_address(sim.param_p)
+        prd = np.array([prm.xprd, prm.yprd, prm.zprd])
+        live = _live_slots(sim.get("ci_natoms"), 4)
+        sx = sim.get("slot_x").reshape(-1, 3)[live]
+        sv = sim.get("slot_v").reshape(-1, 3)[live]
+        st = sim.get("slot_t")[live]
+        wrapped = np.where(sx < 0, sx + prd, np.where(sx >= prd, sx - prd, sx))
+        assert (wrapped != sx).any()  # the wrap has something to do
+        L.updateSingleAtomsGFX950(sim.param_p, sim.atom_p)
+        np.testing.assert_array_equal(sim.get("atoms_x").reshape(-1, 3), sx)
+        np.testing.assert_array_equal(sim.get("atoms_v").reshape(-1, 3), sv)
+        np.testing.assert_array_equal(sim.get("atoms_type"), st)
+        L.updateAtomsPbcClusterGFX950(sim.atom_p, sim.param_p, False)
+        np.testing.assert_array_equal(sim.get("atoms_x").reshape(-1, 3), wrapped)
+        L.updateSingleAtomsGFX950(sim.param_p, sim.atom_p)
+        L.updateAtomsPbcClusterGFX950(sim.atom_p, sim.param_p, False)
+        np.testing.assert_array_equal(sim.get("atoms_x").reshape(-1, 3), wrapped)
+        np.testing.assert_array_equal(sim.get("atoms_v").reshape(-1, 3), sv)
+    finally:
+        sim.close()
+
+
 @pytest.mark.parametrize("nx", [3, 5])
 def test_fused_run_in_small_boxes_equals_unfused_run(nx):
     """Boxes so small that a cluster lies within the list cutoff of both faces of an axis (3
