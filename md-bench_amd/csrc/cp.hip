@@ -954,7 +954,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
     const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
     const int* __restrict__ jstart, const int* __restrict__ jlist,
     const int* __restrict__ zidx, double z0, double inv_dz, const double* __restrict__ sx,
-    double cutneighsq, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
+    double cutneighsq, double R, double rbb_sq, const unsigned long long* __restrict__ hmax_bits,
     int maxneighs, int* __restrict__ nb, int* __restrict__ numneigh, int* __restrict__ nmasked,
     int* __restrict__ status)
 {
@@ -983,8 +983,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
 #pragma unroll
         for (int d = 0; d < 3; d++)
             xi[cii][d] = sx[3 * (ci * M + cii) + d];
-    // search window in bbminz; the margin only widens it
-    const double R    = sqrt(cutneighsq) * (1.0 + 1e-12) + 1e-9;
+    // search window in bbminz (R: the list cutoff with a margin, from the host; the margin only
+    // widens the window)
     const double hmax = __longlong_as_double((long long)*hmax_bits);
     const double zlo  = ib[4] - R - hmax * (1.0 + 1e-12) - 1e-9;
     const double zhi  = ib[5] + R;
@@ -3065,6 +3065,7 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
     if (ncj_all > 0)
         k_cp_jheight<<<std::min(blocks_for(ncj_all, BLOCK), 512u), BLOCK, 0, s>>>(ncj_all,
             C.d_cj_bbox, d_hmax);
+    const double Rwin = sqrt(C.cutneighsq) * (1.0 + 1e-12) + 1e-9;
     for (int attempt = 0; attempt < 8; attempt++) {
         grow_n(&C.d_nb, &C.cap_nb, (size_t)nci * C.maxneighs + 1, 0);
         grow_n(&C.d_numneigh, &C.cap_numneigh, (size_t)nci + 1, 0);
@@ -3075,7 +3076,8 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
 #define MDG_NB(MM, NN)                                                                           \
     k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
         C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_jkey, C.d_jstart,         \
-        C.d_jlist, C.d_zidx, C.z0, C.inv_dz, C.d_sx, C.cutneighsq, C.rbb_sq, d_hmax, C.maxneighs,  \
+        C.d_jlist, C.d_zidx, C.z0, C.inv_dz, C.d_sx, C.cutneighsq, Rwin, C.rbb_sq, d_hmax,        \
+        C.maxneighs,                                                                             \
         C.d_nb,                                                                                  \
         C.d_numneigh,                                                                            \
         C.d_nmasked, C.d_status)
