@@ -68,7 +68,10 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
    updateAtomsPbcCPU (pbc.c:120-145) */
 void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate);
 void updateAtomsPbcClusterGFX950(Atom* atom, Parameter* param, bool dummy);
-/* replaces updateSingleAtoms (neighbor.c:2096-2150): clusters -> atoms in cluster order */
+/* replaces updateSingleAtoms (neighbor.c:2096-2150): clusters -> atoms in cluster order.
+   This and updateAtomsPbcClusterGFX950 are deferred: the next buildClustersGFX950 runs both in
+   its first kernel; any other call that reads or writes the atoms or the slots first runs what
+   is pending, so every call sees the state the two would have left. */
 void updateSingleAtomsGFX950(Parameter* param, Atom* atom);
 /* replaces computeForceLJCuda (force_lj_cuda.cu:603-700) and computeForceLJRef /
    computeForceLJ4xn{Full,Half}Neigh (src/clusterpair/force_lj.c): full or half list by
