@@ -1580,12 +1580,13 @@ __global__ void __launch_bounds__(BLOCK) k_cp_force_f32(int nci, int ncj, int du
 //   z_a - y: three subtractions per pair;
 // - type-table variants (the Kob-Andersen mixture, whose force error the frame's extra
 //   rounding pushed past 1e-6): d = (k_a - k_j) h + (o_a - o_j) per pair, the cell difference
-//   exact and one rounding at d, nine operations per pair; the j type is the record's fourth
+//   exact and one rounding at d, nine operations per pair (the cells held as FP32 values:
+//   one conversion per j-atom, one subtraction per pair); the j type is the record's fourth
 //   byte.
-// The one-type variant is held to 80 VGPRs, six wavefronts per SIMD, without spills; the
-// type-table, energy and compensated variants keep their register count.
+// The one-type variant is held to 80 VGPRs, six wavefronts per SIMD, the type-table variant to
+// 96, five, both without spills; the energy and compensated variants keep their register count.
 template <int N, bool EV, bool UNIFORM, bool COMP>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((UNIFORM && !COMP && !EV) ? 6 : 1)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((COMP || EV) ? 1 : (UNIFORM ? 6 : 5))))
 k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     int ntypes, Tab32 tab, const double* __restrict__ sx, const float4* __restrict__ sr,
     const int* __restrict__ st, const int* __restrict__ nb, const int* __restrict__ numneigh,
@@ -1659,6 +1660,9 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
             zi[a][2] = fmaf(ki[a][2] - kI[2], hf, r.z);
         }
         ti[a] = UNIFORM ? 0 : (int)(w >> 24);
+        // type-table: the cell indices stay FP32 values (opaque to the compiler, which would
+        // otherwise turn each pair's ki - kj back into a byte subtraction and a conversion)
+        if (SPLIT) asm volatile("" : "+v"(ki[a][0]), "+v"(ki[a][1]), "+v"(ki[a][2]));
     }
     const int nn = live ? numneigh[ci] : 0;
     // the group's atom count, for the tail, loaded here rather than after the loop
@@ -1682,8 +1686,10 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
     auto use = [&](const JL& J, auto self) {
         constexpr bool SELF = decltype(self)::value;
         const unsigned wj = __float_as_uint(J.r.w);
-        const float kx = (float)(wj & 0xffu), ky = (float)((wj >> 8) & 0xffu),
-                    kz = (float)((wj >> 16) & 0xffu);
+        float kx = (float)(wj & 0xffu), ky = (float)((wj >> 8) & 0xffu),
+              kz = (float)((wj >> 16) & 0xffu);
+        // one conversion per j-atom, one FP32 subtraction per pair (see the i-atoms' cells)
+        if (SPLIT) asm volatile("" : "+v"(kx), "+v"(ky), "+v"(kz));
         const int tj = UNIFORM ? 0 : (int)(wj >> 24);
         float yx = 0.f, yy = 0.f, yz = 0.f;
         if (!SPLIT) {
