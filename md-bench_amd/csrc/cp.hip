@@ -2380,7 +2380,22 @@ __global__ void k_cp_gather_half(int nslot_loc, int M, const int* __restrict__ c
     const int c = s / N, jj = s - c * N;
     float rx = fown[3 * s], ry = fown[3 * s + 1], rz = fown[3 * s + 2];
     const int e = rstart[c + 1];
-    for (int p = rstart[c]; p < e; p++) {
+    int p       = rstart[c];
+    // four records at a time: their indices, then their values load together; the sums keep
+    // record order
+    for (; p + 4 <= e; p += 4) {
+        const int4 r4 = make_int4(rev[p], rev[p + 1], rev[p + 2], rev[p + 3]);
+        const float* q0 = rec + ((size_t)r4.x * N + jj) * 3;
+        const float* q1 = rec + ((size_t)r4.y * N + jj) * 3;
+        const float* q2 = rec + ((size_t)r4.z * N + jj) * 3;
+        const float* q3 = rec + ((size_t)r4.w * N + jj) * 3;
+        const float a0 = q0[0], b0 = q0[1], c0 = q0[2], a1 = q1[0], b1 = q1[1], c1 = q1[2];
+        const float a2 = q2[0], b2 = q2[1], c2 = q2[2], a3 = q3[0], b3 = q3[1], c3 = q3[2];
+        rx = (((rx + a0) + a1) + a2) + a3;
+        ry = (((ry + b0) + b1) + b2) + b3;
+        rz = (((rz + c0) + c1) + c2) + c3;
+    }
+    for (; p < e; p++) {
         const float* q = rec + ((size_t)rev[p] * N + jj) * 3;
         rx += q[0];
         ry += q[1];
