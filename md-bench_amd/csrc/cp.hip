@@ -948,7 +948,12 @@ __global__ void k_cp_jheight(int n, const double* __restrict__ cj_bbox,
 // more spilled) it took 557 us.  4x8 (143 VGPRs) forced to six spilled its eight j-positions
 // and doubled its time (667 -> 1258 us); it keeps the compiler's choice.
 constexpr int NB_SEG = 256;
-template <int M, int N>
+// FMA: the atom-pair distance formed with fused multiply-adds (1, the 4xN SIMD reference's
+// test) or without (0, the plain-C one), fixed at compile time so that the pair loop has no
+// branch; 2: chosen by fma_dist at run time.  With the fixed form the 4x8 step ran 2-3 %
+// faster; the 4x4 build keeps the run-time flag (fixed, the compiler keeps more values in
+// scalar registers and spills them into the candidate loops: 1 % slower per step).
+template <int M, int N, int FMA>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M == 4 && N == 4) ? 6 : 1))) k_cp_build_nb(int nci, int rule, int fma_dist, int half,
     const int* __restrict__ stencil, int nstencil, int mbins, const int* __restrict__ ci_bin,
     const double* __restrict__ ci_bbox, const double* __restrict__ jbox,
@@ -1078,8 +1083,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((M =
                             double dx = xi[cii][0] - xj[cjj][0];
                             double dy = xi[cii][1] - xj[cjj][1];
                             double dz = xi[cii][2] - xj[cjj][2];
-                            double rsq = fma_dist ? fma(dx, dx, fma(dy, dy, dz * dz))
-                                                  : dx * dx + dy * dy + dz * dz;
+                            const bool fm = FMA == 2 ? fma_dist != 0 : FMA == 1;
+                            double rsq = fm ? fma(dx, dx, fma(dy, dy, dz * dz))
+                                            : dx * dx + dy * dy + dz * dz;
                             pass = pass || rsq < cutneighsq;
                         }
                     s_fl[w][k] = pass ? C_ACCEPT : C_REJECT;
@@ -3094,18 +3100,21 @@ void buildNeighborClusterGFX950(Atom* atom, Neighbor* neighbor)
         if (failed()) return;
         if (attempt > 0) fill_i32(C.d_status, 0, 2);
         unsigned grid = blocks_for(nci, BLOCK / 64);
-#define MDG_NB(MM, NN)                                                                           \
-    k_cp_build_nb<MM, NN><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh,    \
+#define MDG_NB1(MM, NN, FM)                                                                      \
+    k_cp_build_nb<MM, NN, FM><<<grid, BLOCK, 0, s>>>(nci, C.rule, C.fma_dist, neighbor->half_neigh, \
         C.d_stencil, C.nstencil, C.g.mbins, C.d_ci_bin, C.d_ci_bbox, C.d_jkey, C.d_jstart,         \
         C.d_jlist, C.d_zidx, C.z0, C.inv_dz, C.d_sx, C.cutneighsq, Rwin, C.rbb_sq, d_hmax,        \
-        C.maxneighs,                                                                             \
-        C.d_nb,                                                                                  \
-        C.d_numneigh,                                                                            \
-        C.d_nmasked, C.d_status)
-        if (C.M == 4 && C.N == 4) MDG_NB(4, 4);
+        C.maxneighs, C.d_nb, C.d_numneigh, C.d_nmasked, C.d_status)
+#define MDG_NB(MM, NN)                                                                           \
+    do {                                                                                         \
+        if (C.fma_dist) MDG_NB1(MM, NN, 1);                                                      \
+        else MDG_NB1(MM, NN, 0);                                                                 \
+    } while (0)
+        if (C.M == 4 && C.N == 4) MDG_NB1(4, 4, 2);
         else if (C.M == 4 && C.N == 8) MDG_NB(4, 8);
         else MDG_NB(1, 2);
 #undef MDG_NB
+#undef MDG_NB1
         MDG_LAUNCH_CHECK("cp_build_nb");
         // the tile tables of an FP32 half list and the records of the new slots are launched
         // before the status read, which then serves them too (the force call would otherwise
