@@ -713,7 +713,8 @@ __global__ void k_cp_ghost_first(int ncj, int nghost, int N, const int* __restri
 
 // updatePbcCPU with firstUpdate = 0: one thread per ghost slot; with sr, its FP32 record too
 // (the ghost's own, from its shifted FP64 position).
-__global__ void k_cp_ghost_update(int ncj, int nghost, int N, const int* __restrict__ border,
+template <int N>
+__global__ void k_cp_ghost_update(int ncj, int nghost, const int* __restrict__ border,
     const int* __restrict__ pbc, const int* __restrict__ cj_natoms, double xprd, double yprd,
     double zprd, double* __restrict__ sx, const int* __restrict__ st, RecGrid rg,
     float4* __restrict__ sr, int* __restrict__ status)
@@ -721,15 +722,21 @@ __global__ void k_cp_ghost_update(int ncj, int nghost, int N, const int* __restr
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nghost * N) return;
     int g = s / N, cjj = s % N;
-    if (cjj >= cj_natoms[ncj + g]) return;
-    const int sb = border[g] * N + cjj, sg = (ncj + g) * N + cjj;
+    // every load is issued before the atom-count test, whose only effect is on the stores (a
+    // padding slot's source is a padding slot of the source cluster, in bounds): one chain of
+    // two dependent loads per slot instead of three
+    const int na   = cj_natoms[ncj + g];
+    const int sb   = border[g] * N + cjj, sg = (ncj + g) * N + cjj;
+    const int tg   = st[sg];
     const double x = sx[3 * sb] + pbc[3 * g] * xprd;
     const double y = sx[3 * sb + 1] + pbc[3 * g + 1] * yprd;
     const double z = sx[3 * sb + 2] + pbc[3 * g + 2] * zprd;
+    asm volatile("" ::"v"(x), "v"(y), "v"(z), "v"(tg));  // (the compiler would sink the loads)
+    if (cjj >= na) return;
     sx[3 * sg]     = x;
     sx[3 * sg + 1] = y;
     sx[3 * sg + 2] = z;
-    if (sr) sr[sg] = make_rec(x, y, z, st[sg], rg, status);
+    if (sr) sr[sg] = make_rec(x, y, z, tg, rg, status);
 }
 
 // ---------------------------------------------------------------- binJClusters (neighbor.c:1929-2094)
@@ -3015,9 +3022,14 @@ void updatePbcClusterGFX950(Atom* atom, Parameter* param, bool firstUpdate)
         // the records of the ghost slots too, while the other records are current (each
         // record buffer then holds every slot)
         long long n = (long long)C.nghost * C.N;
-        k_cp_ghost_update<<<blocks_for(n, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost, C.N,
-            C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx,
-            C.d_st, C.rg, C.rec_valid ? C.d_sr : nullptr, C.d_status + ST_REC_WORD);
+#define MDG_GU(NN)                                                                               \
+    k_cp_ghost_update<NN><<<blocks_for(n, BLOCK), BLOCK, 0, stream()>>>(C.ncj, C.nghost,         \
+        C.d_border, C.d_pbc, C.d_cj_natoms, param->xprd, param->yprd, param->zprd, C.d_sx, C.d_st, \
+        C.rg, C.rec_valid ? C.d_sr : nullptr, C.d_status + ST_REC_WORD)
+        if (C.N == 4) MDG_GU(4);
+        else if (C.N == 8) MDG_GU(8);
+        else MDG_GU(2);
+#undef MDG_GU
     }
     MDG_LAUNCH_CHECK("cp_update_pbc");
 }
