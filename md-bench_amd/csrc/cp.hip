@@ -1943,6 +1943,17 @@ constexpr int TILE_W64   = 7680;  // table capacity in 64-bit words (60 KB): 640
                                   // 320 of 8x3
 constexpr int TILE_WAVES = 8;     // wavefronts per tile block, four i-clusters each per round
 constexpr double FIX_ONE = 1099511627776.0;  // 2^40, one force unit in table units
+
+// v * 2^40 rounded to the nearest integer (ties to even) as a 64-bit integer: the value of
+// __double2ll_rn((double)v * FIX_ONE), in FP32 operations, each exact (the scalings by powers
+// of two; the low part, which holds at most the 24 bits of y below 2^32); |v| < 2^23.
+__device__ __forceinline__ long long fix40(float v)
+{
+    const float y  = __builtin_rintf(v * 1099511627776.0f);
+    const float hi = floorf(y * 2.3283064365386963e-10f);
+    const float lo = fmaf(-hi, 4294967296.0f, y);
+    return (long long)(int)hi * 4294967296ll + (long long)(unsigned)lo;
+}
 static_assert(TILE_CI % (4 * TILE_WAVES) == 0, "whole rounds per tile");
 
 constexpr int TILE_HASH = 2 * TILE_ECAP;  // open-addressing set of a tile's j-clusters (LDS)
@@ -2294,9 +2305,9 @@ __global__ void __launch_bounds__(TILE_WAVES * 64) k_cp_force_half_tiled(int nci
             }
             if (J.slot >= 0) {
                 unsigned long long* q = s_table + (J.slot * N + jj) * 3;
-                atomicAdd(q, (unsigned long long)__double2ll_rn((double)gx * FIX_ONE));
-                atomicAdd(q + 1, (unsigned long long)__double2ll_rn((double)gy * FIX_ONE));
-                atomicAdd(q + 2, (unsigned long long)__double2ll_rn((double)gz * FIX_ONE));
+                atomicAdd(q, (unsigned long long)fix40(gx));
+                atomicAdd(q + 1, (unsigned long long)fix40(gy));
+                atomicAdd(q + 2, (unsigned long long)fix40(gz));
             }
         };
         auto entry = [&](int k) {
