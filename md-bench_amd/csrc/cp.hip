@@ -740,15 +740,14 @@ __global__ void k_cp_ghost_update(int ncj, int nghost, const int* __restrict__ b
 }
 
 // ---------------------------------------------------------------- binJClusters (neighbor.c:1929-2094)
-__global__ void k_cp_ghost_bin(int ncj, int nghost, int N, CGrid g, const int* __restrict__ cj_natoms,
-    const double* __restrict__ sx, const int* __restrict__ pbc, int* __restrict__ gbin,
-    int* __restrict__ gcnt, int* __restrict__ status)
+// The bin of ghost cluster k (binJClusters' rule for its atoms), -1 for an empty cluster or
+// (flagged) one outside the grid.
+__device__ inline int ghost_bin_of(int ncj, int k, int N, const CGrid& g,
+    const int* __restrict__ cj_natoms, const double* __restrict__ sx,
+    const int* __restrict__ pbc, int* __restrict__ status)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nghost) return;
     int cj = ncj + k, na = cj_natoms[cj];
-    gbin[k] = -1;
-    if (na <= 0) return;
+    if (na <= 0) return -1;
     int ix, iy;
     c2b2d(g, sx[3 * cj * N], sx[3 * cj * N + 1], &ix, &iy);
     ix = max(min(ix, g.mbinx - 1), 0);
@@ -767,11 +766,30 @@ __global__ void k_cp_ghost_bin(int ncj, int nghost, int N, CGrid g, const int* _
     int b = iy * g.mbinx + ix + 1;
     if (b < 0 || b >= g.mbins) {
         atomicOr(status, ST_BIN_RANGE);
-        return;
+        return -1;
     }
-    gbin[k] = b;
-    atomicAdd(&gcnt[b], 1);
+    return b;
 }
+
+__global__ void k_cp_ghost_bin(int ncj, int nghost, int N, CGrid g, const int* __restrict__ cj_natoms,
+    const double* __restrict__ sx, const int* __restrict__ pbc, int* __restrict__ gbin,
+    int* __restrict__ gcnt, int* __restrict__ status)
+{
+    const int k    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int b    = k < nghost ? ghost_bin_of(ncj, k, N, g, cj_natoms, sx, pbc, status) : -1;
+    if (k < nghost) gbin[k] = b;
+    // one atomic per wavefront and bin (consecutive ghosts mostly share a bin)
+    unsigned long long act = __ballot(b >= 0);
+    while (act) {
+        const int l                  = __ffsll((long long)act) - 1;
+        const int lb                 = __shfl(b, l, 64);
+        const unsigned long long grp = __ballot(b == lb);
+        if (lane == l) atomicAdd(&gcnt[lb], __popcll(grp));
+        act &= ~grp;
+    }
+}
+
 
 __global__ void k_cp_jbin_count(int mbins, int M, int N, const int* __restrict__ ncl,
     const int* __restrict__ gcnt, int* __restrict__ jcnt)
@@ -821,11 +839,23 @@ __global__ void __launch_bounds__(ONE_T) k_cp_jbin_starts(int mbins, int M, int 
 __global__ void k_cp_ghost_scatter(int nghost, const int* __restrict__ gbin,
     const int* __restrict__ gstart, int* __restrict__ gcur, int* __restrict__ glist)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nghost) return;
-    int b = gbin[k];
-    if (b < 0) return;
-    glist[gstart[b] + atomicAdd(&gcur[b], 1)] = k;
+    // as k_cp_bin_scatter: one atomic per wavefront and bin (the order inside a bin is the
+    // merge's to fix)
+    const int k    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int b    = k < nghost ? gbin[k] : -1;
+    unsigned long long act = __ballot(b >= 0);
+    while (act) {
+        const int l                  = __ffsll((long long)act) - 1;
+        const int lb                 = __shfl(b, l, 64);
+        const unsigned long long grp = __ballot(b == lb);
+        int base                     = 0;
+        if (lane == l) base = atomicAdd(&gcur[lb], __popcll(grp));
+        base = __shfl(base, l, 64);
+        if (b == lb) glist[gstart[lb] + base + __popcll(grp & lt)] = k;
+        act &= ~grp;
+    }
 }
 
 // One wavefront per bin: locals (in cluster order) then ghosts, stably ordered by bbminz --
