@@ -399,11 +399,20 @@ __global__ void k_cp_bin_scatter(int natoms, const int* __restrict__ bin_of,
 // the unique z order, so each atom's place is its z rank.  Only a column with equal z values (a
 // perfect lattice) restores index order and replays the selection sort, whose swaps decide the
 // order of ties.  Also counts the bin's clusters.
-constexpr int SORT_T = 256;
+// The z rank: the atoms go into SORT_NBK z buckets (floor(z inv_dz), clamped: non-decreasing in
+// z, so an atom in a lower bucket has a smaller z); an atom's rank is the count of the lower
+// buckets plus the atoms of its own bucket with a smaller z (mostly none or one).
+constexpr int SORT_T = 256, SORT_NBK = 1024, SORT_PER = BIN_CAP / SORT_T;
+__device__ inline int zsort_bucket(double z, double inv_dz)
+{
+    const double t = floor(z * inv_dz);
+    return t < 0.0 ? 0 : (t > (double)(SORT_NBK - 1) ? SORT_NBK - 1 : (int)t);
+}
 __global__ void __launch_bounds__(SORT_T) k_cp_bin_sort(const int* __restrict__ start,
     int* __restrict__ atoms, const double* __restrict__ ax, int M, int N, int* __restrict__ ncl,
-    int* __restrict__ status)
+    double inv_dz, int* __restrict__ status)
 {
+    __shared__ int s_bk[SORT_NBK];
     __shared__ __align__(16) int s_idx[BIN_CAP];
     __shared__ __align__(16) int s_tmp[BIN_CAP];
     __shared__ __align__(16) double s_z[BIN_CAP];
@@ -428,32 +437,65 @@ __global__ void __launch_bounds__(SORT_T) k_cp_bin_sort(const int* __restrict__ 
         s_idx[p]    = i;
         s_z[p]      = ax[3 * i + 2];
     }
+    for (int q = t; q < SORT_NBK; q += SORT_T)
+        s_bk[q] = 0;
+    __syncthreads();
+    for (int p = t; p < c; p += SORT_T)
+        atomicAdd(&s_bk[zsort_bucket(s_z[p], inv_dz)], 1);
+    __syncthreads();
+    {  // bucket starts: an exclusive scan, SORT_NBK / SORT_T buckets per thread
+        __shared__ int sh[33];
+        constexpr int PB = SORT_NBK / SORT_T;
+        int v[PB], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PB; k++) {
+            v[k] = s_bk[t * PB + k];
+            sum += v[k];
+        }
+        int tot;
+        int off = mdg::block_excl_scan(sum, sh, &tot);
+#pragma unroll
+        for (int k = 0; k < PB; k++) {
+            s_bk[t * PB + k] = off;
+            off += v[k];
+        }
+    }
+    __syncthreads();
+    // the atoms in bucket order (s_bk then holds each bucket's end)
+    for (int p = t; p < c; p += SORT_T)
+        s_tmp[atomicAdd(&s_bk[zsort_bucket(s_z[p], inv_dz)], 1)] = p;
     __syncthreads();
     bool tie = false;
-    for (int p = t; p < c; p += SORT_T) {
-        const double zp = s_z[p];
-        int r = 0, e = 0, q = 0;
-        // four independent LDS reads in flight per round (the rolled loop waited on each)
-#pragma unroll 4
-        for (; q + 2 <= c; q += 2) {
-            const double2 zq = *reinterpret_cast<const double2*>(&s_z[q]);
-            r += (zq.x < zp) + (zq.y < zp);
-            e += (zq.x == zp) + (zq.y == zp);
+    int rk[SORT_PER];
+#pragma unroll
+    for (int k = 0; k < SORT_PER; k++) {
+        const int p = t + k * SORT_T;
+        rk[k]       = 0;
+        if (p < c) {
+            const double zp = s_z[p];
+            const int bk    = zsort_bucket(zp, inv_dz);
+            const int q1    = s_bk[bk];
+            int r = bk ? s_bk[bk - 1] : 0, e = 0;
+            for (int q = r; q < q1; q++) {
+                const double zq = s_z[s_tmp[q]];
+                r += zq < zp;
+                e += zq == zp;
+            }
+            tie |= e > 1;
+            rk[k] = r;
         }
-        if (q < c) {
-            r += s_z[q] < zp;
-            e += s_z[q] == zp;
-        }
-        tie |= e > 1;
-        s_tmp[p] = r;
     }
     if (__any(tie) && lane == 0) s_tie = 1;
     __syncthreads();
     if (!s_tie) {
-        for (int p = t; p < c; p += SORT_T)
-            atoms[s0 + s_tmp[p]] = s_idx[p];
+#pragma unroll
+        for (int k = 0; k < SORT_PER; k++) {
+            const int p = t + k * SORT_T;
+            if (p < c) atoms[s0 + rk[k]] = s_idx[p];
+        }
         return;
     }
+    __syncthreads();  // (s_tmp is reused below)
     // ties: index order first (indices are distinct), then the selection sort on one wavefront
     for (int p = t; p < c; p += SORT_T) {
         const int v = s_idx[p];
@@ -2964,6 +3006,7 @@ void buildClustersGFX950(Atom* atom)
     k_cp_bin_scatter<<<blocks_for(na, BLOCK), BLOCK, 0, s>>>(na, C.d_abin_of, C.d_abin_start,
         C.d_abin_cnt, C.d_abin_atoms);
     k_cp_bin_sort<<<mb, SORT_T, 0, s>>>(C.d_abin_start, C.d_abin_atoms, C.d_ax, C.M, C.N, C.d_ncl,
+        (double)SORT_NBK / C.g.zprd,
         C.d_status);
     // the cluster starts and the cluster count (to status word 8)
     exclusive_scan(C.d_ncl, C.d_cstart, mb, C.d_cstart + mb, C.d_status + 8);
