@@ -130,7 +130,7 @@ struct CPState {
     // false when a tile outgrew its tables (that list then runs the atomic kernel)
     bool tiles_valid = false, tiles_ok = false;
     bool tiles_wait = false;  // tile tables launched, their status not yet read
-    int ntiles = 0, tile_ucap = 0;
+    int ntiles = 0, tile_ucap = 0, tile_nmax = 0;
     int *d_tile_cj = nullptr, *d_tile_n = nullptr, *d_eslot = nullptr, *d_rec_off = nullptr,
         *d_rcount = nullptr, *d_rstart = nullptr, *d_rcur = nullptr, *d_rev = nullptr;
     size_t cap_tile_cj = 0, cap_tile_n = 0, cap_eslot = 0, cap_rec_off = 0, cap_rcount = 0,
@@ -2009,11 +2009,12 @@ k_cp_force_f32_full(int nci, int ncj, int dummy_cj,
 // point (2^-40 units, range +-2^23): integer adds are exact, so the sums do not depend on the
 // order in which the wavefronts of a tile add (and LDS integer atomics run at full rate, FP32
 // ones did not: 0.57 against 0.21 ms per 1M-atom call).  No global atomics anywhere.
-constexpr int TILE_CI    = 32;    // i-clusters per tile
+constexpr int TILE_CI    = 16;    // i-clusters per tile
 constexpr int TILE_ECAP  = 2048;  // list entries per tile (more in any tile: the atomic kernel)
 constexpr int TILE_W64   = 7680;  // table capacity in 64-bit words (60 KB): 640 slots of 4x3,
-                                  // 320 of 8x3
-constexpr int TILE_WAVES = 8;     // wavefronts per tile block, four i-clusters each per round
+                                  // 320 of 8x3 (the force pass's LDS holds the largest table
+                                  // of the current list, not the capacity)
+constexpr int TILE_WAVES = 4;     // wavefronts per tile block, four i-clusters each per round
 constexpr double FIX_ONE = 1099511627776.0;  // 2^40, one force unit in table units
 
 // v * 2^40 rounded to the nearest integer (ties to even) as a 64-bit integer: the value of
@@ -2033,7 +2034,7 @@ constexpr int TILE_TT   = 256;   // threads of the tile-table kernel
 __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int ucap,
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
     int* __restrict__ tile_cj, int* __restrict__ tile_n, int* __restrict__ entry_slot,
-    int* __restrict__ status)
+    int* __restrict__ status, int* __restrict__ nmax)
 {
     static_assert(TILE_CI <= 64, "the list-length prefix is one wavefront scan");
     __shared__ int s_hash[TILE_HASH];
@@ -2142,7 +2143,10 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         }
     for (int u = tid; u < tot; u += TILE_TT)
         tile_cj[(size_t)t * ucap + u] = s_uni[u];
-    if (tid == 0) tile_n[t] = tot;
+    if (tid == 0) {
+        tile_n[t] = tot;
+        atomicMax(nmax, tot);  // the largest table, which sizes the force pass's LDS
+    }
     // every entry's slot: its j-cluster's place in the table (binary search)
     for (int e = tid; e < E; e += TILE_TT) {
         const size_t at = entry_at(e);
@@ -3270,6 +3274,7 @@ void pruneNeighborClusterGFX950(Parameter* param, Atom* atom, Neighbor* neighbor
 // Tile tables and the reverse map of the current half list (see k_cp_tile_table).  A tile that
 // outgrows its tables leaves tiles_ok false: that list runs the atomic half-list kernel.
 constexpr int ST_TILE_WORD = 15;  // status word of the tile-table kernel
+constexpr int ST_TILE_NMAX = 11;  // the largest tile table of the current list (slots)
 // The tile-table kernel of the current list, before a status read: the list build and the
 // prune launch it before their own status read, so that one read serves both.
 static void tiles_launch()
@@ -3295,9 +3300,9 @@ static void tiles_launch()
     grow_n(&C.d_rec, &C.cap_rec, nrec * 3 * N, 0);
     grow_n(&C.d_fown, &C.cap_fown, 3 * (size_t)nci * C.M, 0);
     if (failed()) return;
-    fill_i32(C.d_status + ST_TILE_WORD, 0, 1);
+    fill_i32_multi({ { C.d_status + ST_TILE_WORD, 0, 1 }, { C.d_status + ST_TILE_NMAX, 0, 1 } });
     k_cp_tile_table<<<ntiles, TILE_TT, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
-        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD);
+        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD, C.d_status + ST_TILE_NMAX);
     MDG_LAUNCH_CHECK("cp_tile_table");
     C.tiles_wait = true;
 }
@@ -3309,6 +3314,7 @@ static void tiles_finish()
     if (!C.tiles_wait) return;
     C.tiles_wait = false;
     if (failed() || C.h_status[ST_TILE_WORD]) return;
+    C.tile_nmax = C.h_status[ST_TILE_NMAX];
     hipStream_t s = stream();
     const int ntiles = C.ntiles, ucap = C.tile_ucap, ncj = C.ncj;
     const size_t nrec = (size_t)ntiles * ucap;
@@ -3517,7 +3523,8 @@ static int force_cluster(Parameter* param, Neighbor* neighbor, bool fuse_final,
         }                                                                                        \
     } while (0)
         if (tiled) {
-            const size_t lds = (size_t)C.tile_ucap * 3 * C.N * sizeof(unsigned long long);
+            // the kernel clears and uses the first tile_n[t] slots of its table
+            const size_t lds = (size_t)std::max(C.tile_nmax, 1) * 3 * C.N * sizeof(unsigned long long);
             const unsigned gg = blocks_for((long long)nloc, BLOCK);
 #define MDG_HT(NN, E, U)                                                                         \
     k_cp_force_half_tiled<NN, E, U><<<C.ntiles, TILE_WAVES * 64, lds, s>>>(nci, C.ncj, C.dummy_cj, \
