@@ -2030,6 +2030,8 @@ __device__ __forceinline__ long long fix40(float v)
 static_assert(TILE_CI % (4 * TILE_WAVES) == 0, "whole rounds per tile");
 
 constexpr int TILE_HASH = 2 * TILE_ECAP;  // open-addressing set of a tile's j-clusters (LDS)
+constexpr int TILE_UNI  = 2048;  // the sorted distinct j-clusters, padded to a power of two
+static_assert(TILE_W64 / (3 * 2) <= TILE_UNI, "a table of 2-atom j-clusters sorts in s_uni");
 constexpr int TILE_TT   = 256;   // threads of the tile-table kernel
 __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int ucap,
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
@@ -2038,13 +2040,11 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
 {
     static_assert(TILE_CI <= 64, "the list-length prefix is one wavefront scan");
     __shared__ int s_hash[TILE_HASH];
-    __shared__ int s_uni[TILE_HASH];
+    __shared__ int s_uni[TILE_UNI];
     __shared__ int s_pre[TILE_CI + 1];
     __shared__ int s_sh[33];
     const int t = blockIdx.x, c0 = t * TILE_CI, nc = min(nci - c0, TILE_CI);
     const int tid = threadIdx.x;
-    for (int k = tid; k < TILE_HASH; k += TILE_TT)
-        s_hash[k] = -1;
     // entries of the tile: prefix of the list lengths (clamped: the kernel may run on a list
     // whose build overflowed, before the host has seen that and rebuilt it)
     if (tid < 64) {
@@ -2063,6 +2063,13 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         if (tid == 0) atomicOr(status, 1);
         return;
     }
+    // the set's size: the power of two at least twice the entry count
+    int HS = 64;
+    while (HS < 2 * E)
+        HS <<= 1;
+    for (int k = tid; k < HS; k += TILE_TT)
+        s_hash[k] = -1;
+    __syncthreads();
     auto entry_at = [&](int e) {
         int c = 0;
         for (int step = TILE_CI / 2; step > 0; step >>= 1)
@@ -2070,23 +2077,23 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         return (size_t)(c0 + c) * maxneighs + (e - s_pre[c]);
     };
     // the distinct local j-clusters into the set (linear probing; at most half full, as
-    // E <= TILE_ECAP = TILE_HASH / 2, so every probe sequence ends)
+    // 2 E <= HS <= TILE_HASH, so every probe sequence ends)
     for (int e = tid; e < E; e += TILE_TT) {
         const int cj = nb[entry_at(e)];
         if (cj >= ncj) continue;
-        unsigned h = ((unsigned)cj * 2654435761u) & (TILE_HASH - 1);
+        unsigned h = ((unsigned)cj * 2654435761u) & (HS - 1);
         for (;;) {
             const int old = atomicCAS(&s_hash[h], -1, cj);
             if (old == -1 || old == cj) break;
-            h = (h + 1) & (TILE_HASH - 1);
+            h = (h + 1) & (HS - 1);
         }
     }
     __syncthreads();
     // compaction: each thread a contiguous run of the set, block scan of the counts
-    constexpr int RUN = TILE_HASH / TILE_TT;
+    const int RUN = (HS + TILE_TT - 1) / TILE_TT, r0 = min(tid * RUN, HS), r1 = min(r0 + RUN, HS);
     int cnt = 0;
-    for (int q = 0; q < RUN; q++)
-        cnt += s_hash[tid * RUN + q] >= 0;
+    for (int q = r0; q < r1; q++)
+        cnt += s_hash[q] >= 0;
     int tot;
     {
         const int lane = tid & 63, w = tid >> 6, nw = TILE_TT >> 6;
@@ -2110,8 +2117,8 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         __syncthreads();
         int off = inc - cnt + s_sh[w];
         tot     = s_sh[32];
-        for (int q = 0; q < RUN; q++) {
-            const int k = s_hash[tid * RUN + q];
+        for (int q = r0; q < r1; q++) {
+            const int k = s_hash[q];
             if (k >= 0) s_uni[off++] = k;
         }
     }
@@ -2167,21 +2174,22 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
 
 // records of every local j-cluster: count, then (after a scan) fill and sort each run so the
 // gather visits a cluster's records in increasing record order
-__global__ void k_cp_rev_count(int ntiles, int ucap, const int* __restrict__ tile_cj,
+// (w threads per tile: w is the largest table of the list)
+__global__ void k_cp_rev_count(int ntiles, int ucap, int w, const int* __restrict__ tile_cj,
     const int* __restrict__ tile_n, int* __restrict__ rcount)
 {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = (int)(k / ucap), u = (int)(k % ucap);
+    const int t = (int)(k / w), u = (int)(k % w);
     if (t >= ntiles || u >= tile_n[t]) return;
     atomicAdd(&rcount[tile_cj[(size_t)t * ucap + u]], 1);
 }
 
-__global__ void k_cp_rev_fill(int ntiles, int ucap, const int* __restrict__ tile_cj,
+__global__ void k_cp_rev_fill(int ntiles, int ucap, int w, const int* __restrict__ tile_cj,
     const int* __restrict__ tile_n, const int* __restrict__ rec_off,
     const int* __restrict__ rstart, int* __restrict__ rcur, int* __restrict__ rev)
 {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = (int)(k / ucap), u = (int)(k % ucap);
+    const int t = (int)(k / w), u = (int)(k % w);
     if (t >= ntiles || u >= tile_n[t]) return;
     const int cj = tile_cj[(size_t)t * ucap + u];
     rev[rstart[cj] + atomicAdd(&rcur[cj], 1)] = rec_off[t] + u;
@@ -3317,14 +3325,15 @@ static void tiles_finish()
     C.tile_nmax = C.h_status[ST_TILE_NMAX];
     hipStream_t s = stream();
     const int ntiles = C.ntiles, ucap = C.tile_ucap, ncj = C.ncj;
-    const size_t nrec = (size_t)ntiles * ucap;
+    const int w = std::max(C.tile_nmax, 1);  // threads per tile of the reverse-map kernels
+    const size_t nrec = (size_t)ntiles * w;
     exclusive_scan(C.d_tile_n, C.d_rec_off, ntiles, C.d_rec_off + ntiles);
     fill_i32(C.d_rcount, 0, (size_t)ncj + 1);
     fill_i32(C.d_rcur, 0, (size_t)ncj + 1);
-    k_cp_rev_count<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap,
+    k_cp_rev_count<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap, w,
         C.d_tile_cj, C.d_tile_n, C.d_rcount);
     exclusive_scan(C.d_rcount, C.d_rstart, ncj, C.d_rstart + ncj);
-    k_cp_rev_fill<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap,
+    k_cp_rev_fill<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap, w,
         C.d_tile_cj, C.d_tile_n, C.d_rec_off, C.d_rstart, C.d_rcur, C.d_rev);
     k_cp_rev_sort<<<blocks_for(ncj, BLOCK), BLOCK, 0, s>>>(ncj, C.d_rstart, C.d_rev);
     MDG_LAUNCH_CHECK("cp_rev_map");
