@@ -15,7 +15,7 @@
 //                           16-byte FP32 records (offset in a power-of-two grid cell, cell bytes
 //                           and type), the cutoff decided as in FP64 (a band re-check); with the
 //                           run loop's final kick, next half kick and drift fused in.
-//   k_cp_force_half_tiled   FP32-mixed, half lists: one block per tile of 32 i-clusters, the
+//   k_cp_force_half_tiled   FP32-mixed, half lists: one block per tile of 16 i-clusters, the
 //   + k_cp_gather_half      tile's j-forces summed in LDS as 64-bit fixed point and written once
 //                           per tile, then added per atom in record order (no atomics).
 //   k_cp_force_f32          FP32-mixed, one wavefront per i-cluster of 4 atoms (lane = i-atom x
