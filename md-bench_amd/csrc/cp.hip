@@ -2195,11 +2195,47 @@ __global__ void k_cp_rev_fill(int ntiles, int ucap, int w, const int* __restrict
     rev[rstart[cj] + atomicAdd(&rcur[cj], 1)] = rec_off[t] + u;
 }
 
+// A run of at most R records sorted in registers: R independent loads (padded with INT_MAX),
+// a bitonic network with compile-time indices, the n values stored back.
+template <int R>
+__device__ __forceinline__ void rev_sort_regs(int* __restrict__ run, int n)
+{
+    int v[R];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        v[i] = i < n ? run[i] : INT_MAX;
+#pragma unroll
+    for (int k = 2; k <= R; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        if (i < n) run[i] = v[i];
+}
 __global__ void k_cp_rev_sort(int ncj, const int* __restrict__ rstart, int* __restrict__ rev)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncj) return;
     const int b = rstart[c], e = rstart[c + 1];
+    // most runs hold ~10 records; longer ones (rare) by insertion in memory
+    if (e - b <= 16) {
+        rev_sort_regs<16>(rev + b, e - b);
+        return;
+    }
+    if (e - b <= 32) {
+        rev_sort_regs<32>(rev + b, e - b);
+        return;
+    }
     for (int i = b + 1; i < e; i++) {
         const int v = rev[i];
         int j       = i - 1;
