@@ -2030,8 +2030,8 @@ __device__ __forceinline__ long long fix40(float v)
 static_assert(TILE_CI % (4 * TILE_WAVES) == 0, "whole rounds per tile");
 
 constexpr int TILE_HASH = 2 * TILE_ECAP;  // open-addressing set of a tile's j-clusters (LDS)
-constexpr int TILE_UNI  = 2048;  // the sorted distinct j-clusters, padded to a power of two
-static_assert(TILE_W64 / (3 * 2) <= TILE_UNI, "a table of 2-atom j-clusters sorts in s_uni");
+constexpr int TILE_UNI  = TILE_ECAP;  // the distinct j-clusters, at most one per entry
+static_assert(TILE_W64 / (3 * 2) <= TILE_HASH, "the sorted table fits the set's array");
 constexpr int TILE_TT   = 256;   // threads of the tile-table kernel
 __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int ucap,
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
@@ -2126,30 +2126,21 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         if (tid == 0) atomicOr(status, 2);
         return;
     }
-    // sort the distinct keys (bitonic over the next power of two, padded with INT_MAX)
-    int P = 64;
-    while (P < tot)
-        P <<= 1;
+    // the distinct keys in increasing order: each key goes to its rank, the number of keys below
+    // it (every thread reads the same key at once, a broadcast; no barrier per step as a
+    // sorting network needs), into the set's array, which the compaction no longer reads
     __syncthreads();
-    for (int k = tot + tid; k < P; k += TILE_TT)
-        s_uni[k] = INT_MAX;
+    int* s_srt = s_hash;
+    for (int u = tid; u < tot; u += TILE_TT) {
+        const int key = s_uni[u];
+        int r         = 0;
+        for (int q = 0; q < tot; q++)
+            r += s_uni[q] < key;
+        s_srt[r] = key;
+    }
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < P; i += TILE_TT) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const int a = s_uni[i], b = s_uni[l];
-                    if (((i & k) == 0) == (a > b)) {
-                        s_uni[i] = b;
-                        s_uni[l] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
     for (int u = tid; u < tot; u += TILE_TT)
-        tile_cj[(size_t)t * ucap + u] = s_uni[u];
+        tile_cj[(size_t)t * ucap + u] = s_srt[u];
     if (tid == 0) {
         tile_n[t] = tot;
         atomicMax(nmax, tot);  // the largest table, which sizes the force pass's LDS
@@ -2163,7 +2154,7 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
             int lo = 0, hi = tot - 1;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (s_uni[mid] < cj) lo = mid + 1;
+                if (s_srt[mid] < cj) lo = mid + 1;
                 else hi = mid;
             }
             slot = lo;
