@@ -2036,7 +2036,7 @@ constexpr int TILE_TT   = 256;   // threads of the tile-table kernel
 __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int ucap,
     const int* __restrict__ nb, const int* __restrict__ numneigh, int maxneighs,
     int* __restrict__ tile_cj, int* __restrict__ tile_n, int* __restrict__ entry_slot,
-    int* __restrict__ status, int* __restrict__ nmax)
+    int* __restrict__ status, int* __restrict__ nmax, int* __restrict__ rcount)
 {
     static_assert(TILE_CI <= 64, "the list-length prefix is one wavefront scan");
     __shared__ int s_hash[TILE_HASH];
@@ -2139,8 +2139,11 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
         s_srt[r] = key;
     }
     __syncthreads();
-    for (int u = tid; u < tot; u += TILE_TT)
+    // the table, and each j-cluster's record count for the reverse map (cleared before the launch)
+    for (int u = tid; u < tot; u += TILE_TT) {
         tile_cj[(size_t)t * ucap + u] = s_srt[u];
+        atomicAdd(&rcount[s_srt[u]], 1);
+    }
     if (tid == 0) {
         tile_n[t] = tot;
         atomicMax(nmax, tot);  // the largest table, which sizes the force pass's LDS
@@ -2163,18 +2166,9 @@ __global__ void __launch_bounds__(TILE_TT) k_cp_tile_table(int nci, int ncj, int
     }
 }
 
-// records of every local j-cluster: count, then (after a scan) fill and sort each run so the
-// gather visits a cluster's records in increasing record order
-// (w threads per tile: w is the largest table of the list)
-__global__ void k_cp_rev_count(int ntiles, int ucap, int w, const int* __restrict__ tile_cj,
-    const int* __restrict__ tile_n, int* __restrict__ rcount)
-{
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = (int)(k / w), u = (int)(k % w);
-    if (t >= ntiles || u >= tile_n[t]) return;
-    atomicAdd(&rcount[tile_cj[(size_t)t * ucap + u]], 1);
-}
-
+// records of every local j-cluster (counted by the tile-table kernel): after a scan of the
+// counts, fill and sort each run so the gather visits a cluster's records in increasing record
+// order (w threads per tile: w is the largest table of the list)
 __global__ void k_cp_rev_fill(int ntiles, int ucap, int w, const int* __restrict__ tile_cj,
     const int* __restrict__ tile_n, const int* __restrict__ rec_off,
     const int* __restrict__ rstart, int* __restrict__ rcur, int* __restrict__ rev)
@@ -3335,9 +3329,11 @@ static void tiles_launch()
     grow_n(&C.d_rec, &C.cap_rec, nrec * 3 * N, 0);
     grow_n(&C.d_fown, &C.cap_fown, 3 * (size_t)nci * C.M, 0);
     if (failed()) return;
-    fill_i32_multi({ { C.d_status + ST_TILE_WORD, 0, 1 }, { C.d_status + ST_TILE_NMAX, 0, 1 } });
+    fill_i32_multi({ { C.d_status + ST_TILE_WORD, 0, 1 }, { C.d_status + ST_TILE_NMAX, 0, 1 },
+        { C.d_rcount, 0, (size_t)ncj + 1 } });
     k_cp_tile_table<<<ntiles, TILE_TT, 0, s>>>(nci, ncj, ucap, C.d_nb, C.d_numneigh, C.maxneighs,
-        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD, C.d_status + ST_TILE_NMAX);
+        C.d_tile_cj, C.d_tile_n, C.d_eslot, C.d_status + ST_TILE_WORD, C.d_status + ST_TILE_NMAX,
+        C.d_rcount);
     MDG_LAUNCH_CHECK("cp_tile_table");
     C.tiles_wait = true;
 }
@@ -3355,10 +3351,7 @@ static void tiles_finish()
     const int w = std::max(C.tile_nmax, 1);  // threads per tile of the reverse-map kernels
     const size_t nrec = (size_t)ntiles * w;
     exclusive_scan(C.d_tile_n, C.d_rec_off, ntiles, C.d_rec_off + ntiles);
-    fill_i32(C.d_rcount, 0, (size_t)ncj + 1);
     fill_i32(C.d_rcur, 0, (size_t)ncj + 1);
-    k_cp_rev_count<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap, w,
-        C.d_tile_cj, C.d_tile_n, C.d_rcount);
     exclusive_scan(C.d_rcount, C.d_rstart, ncj, C.d_rstart + ncj);
     k_cp_rev_fill<<<blocks_for((long long)nrec, BLOCK), BLOCK, 0, s>>>(ntiles, ucap, w,
         C.d_tile_cj, C.d_tile_n, C.d_rec_off, C.d_rstart, C.d_rcur, C.d_rev);
